@@ -1,0 +1,214 @@
+/*
+ * p9idg -- C ABI of the MI355X (gfx950) image-domain gridder with w-stacking.
+ *
+ * Drop-in boundary for the gridding API of the reference C# code
+ * (paths relative to Distributed-Reconstruction/):
+ *
+ *   GriddingConstants ctor        Core/ImageDomainGridder/GriddingConstants.cs:33
+ *   Subgrid                       Core/ImageDomainGridder/Subgrid.cs:15-24
+ *   Partitioner.CreatePartition   Core/ImageDomainGridder/Partitioner.cs:20
+ *   IDG.Grid / DeGrid             Core/ImageDomainGridder/IDG.cs:12 / :22
+ *   IDG.GridW / DeGridW           Core/ImageDomainGridder/IDG.cs:32 / :42
+ *   IDG.GridPSF / ToImage / CalculatePSF   IDG.cs:52 / :68 / :85
+ *   FFT.WStackIFFTFloat           Core/FFT.cs:188
+ *   FFT.Forward(float[,])         Core/FFT.cs:99
+ *   FFT.Backward(Complex[,],long) Core/FFT.cs:159
+ *   FFT.Shift                     Core/FFT.cs:221, :248
+ *   Visibilities.Substract        Core/Common.cs:305
+ *
+ * Conventions
+ *  - Layouts are those of the C# arrays: row-major, contiguous.
+ *      uvw   double[B][T][3] (metres)      freq double[C] (Hz)
+ *      vis   complex[B][T][C]              grid complex[G][G]
+ *      layers complex[L][G][G]             image float/double[G][G]
+ *    "complex128" is {double re, double im} (System.Numerics.Complex);
+ *    "complex64" is {float re, float im}.
+ *  - Subgrid metadata is the flattening of List<List<Subgrid>>: all subgrids of
+ *    baseline 0 in time order, then baseline 1, ...; baseline_offsets[b] is the
+ *    index of the first subgrid of baseline b (B+1 entries).
+ *  - The caller owns every host array; the plan owns device memory, the HIP
+ *    stream and the FFT plans.  The library never frees caller memory.
+ *  - Every entry point returns a p9idg_status; on failure p9idg_last_error()
+ *    returns the message of the calling thread.  The three reference
+ *    exceptions keep their exact texts.
+ *  - Host-buffer entry points are synchronous.  Device entry points
+ *    (p9idg_exec_*) are asynchronous on the plan's stream; p9idg_sync waits.
+ *  - A plan is not re-entrant; separate plans may be used from separate
+ *    threads concurrently.
+ *  - The degrid entry points read the caller's grid without modifying it.
+ *    (The reference's IDG.DeGrid/DeGridW swap the quadrants of the caller's
+ *    grid in place, IDG.cs:24,44; no caller in the reference reads the grid
+ *    afterwards.)
+ */
+#ifndef P9IDG_H
+#define P9IDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define P9IDG_API_VERSION 1
+
+/* GriddingConstants.cs:11-53.  scale_arcsec is radians per pixel (the
+ * reference passes cellSize in radians, RunningMethod.cs:22,45).
+ * wstep_lambda must be 0: the reference never assigns WStepLambda
+ * (GriddingConstants.cs:25), so its w offset is always 0. */
+typedef struct p9idg_constants {
+    int64_t visibilities_count; /* VisibilitiesCount (normalises the image) */
+    int32_t grid_size;          /* GridSize (even) */
+    int32_t subgrid_size;       /* SubgridSize: 8, 16, 24, 32, 48 or 64 */
+    int32_t kernel_size;        /* KernelSize */
+    int32_t max_timesteps_per_subgrid;
+    float scale_arcsec;
+    int32_t wlayer_count;       /* WLayerCount */
+    double wstep;               /* WStep (lambda); 0 disables w binning */
+    float wstep_lambda;         /* must be 0 */
+} p9idg_constants;
+
+/* Subgrid.cs:15-24 */
+typedef struct p9idg_subgrid {
+    int32_t baseline_idx;
+    int32_t time_sample_start;
+    int32_t time_sample_count;
+    int32_t u_pixel;
+    int32_t v_pixel;
+    int32_t w_lambda; /* w-layer index, may be negative */
+} p9idg_subgrid;
+
+typedef enum p9idg_status {
+    P9IDG_OK = 0,
+    P9IDG_ERR_PLACE = 1,          /* Partitioner.cs:86,91 */
+    P9IDG_ERR_SUBGRID_RANGE = 2,  /* Partitioner.cs:118 */
+    P9IDG_ERR_INVALID_GRID = 3,   /* Adder.cs:49 */
+    P9IDG_ERR_ARGUMENT = 4,
+    P9IDG_ERR_CAPACITY = 5,
+    P9IDG_ERR_HIP = 6,
+    P9IDG_ERR_ALLOC = 7,
+    P9IDG_ERR_STATE = 8,
+    P9IDG_ERR_UNSUPPORTED = 9
+} p9idg_status;
+
+typedef enum p9idg_vis_type { P9IDG_C128 = 0, P9IDG_C64 = 1 } p9idg_vis_type;
+
+/* Phasor arithmetic of the gridder and degridder. */
+typedef enum p9idg_kernel {
+    P9IDG_KERNEL_FP32 = 0,      /* per-pixel phasors on the FP32 vector ALUs */
+    P9IDG_KERNEL_MFMA_BF16 = 1, /* split hi/lo BF16 MFMA contraction, FP32 accumulate */
+    P9IDG_KERNEL_MFMA_FP16 = 2  /* split hi/lo FP16 MFMA contraction, FP32 accumulate */
+} p9idg_kernel;
+
+/* Where p9idg_exec_grid puts the subgrids. */
+typedef enum p9idg_grid_mode {
+    P9IDG_GRID_PLAIN = 0,  /* Adder.Add onto one grid (IDG.Grid), Adder.cs:13 */
+    P9IDG_GRID_WSTACK = 1, /* AdderWStack.Add onto L layers (IDG.GridW), AdderWStack.cs:11 */
+    P9IDG_GRID_WSUM = 2    /* AdderWStack.Add with every layer summed into one grid:
+                              same real image after the inverse FFT, 1 FFT instead of L */
+} p9idg_grid_mode;
+
+/* Where p9idg_exec_degrid reads the subgrids from. */
+typedef enum p9idg_degrid_mode {
+    P9IDG_DEGRID_PLAIN = 0, /* Adder.Split (IDG.DeGrid), Adder.cs:57 */
+    P9IDG_DEGRID_WSTACK = 1 /* AdderWStack.Split (IDG.DeGridW), AdderWStack.cs:81 */
+} p9idg_degrid_mode;
+
+typedef struct p9idg_plan p9idg_plan;
+
+int p9idg_api_version(void);
+const char* p9idg_last_error(void);
+
+/* ---- Partitioner (host, bit-exact with Partitioner.cs:20-124) ----------- */
+/* out may be NULL to query the count.  baseline_offsets may be NULL. */
+int p9idg_partition(const p9idg_constants* c, const double* uvw, int32_t nr_baselines,
+                    int32_t nr_timesteps, const double* freq, int32_t nr_channels,
+                    p9idg_subgrid* out, int64_t capacity, int64_t* nr_subgrids,
+                    int64_t* baseline_offsets);
+
+/* ---- Plan (handle) ------------------------------------------------------- */
+/* Replaces constructing GriddingConstants; owns device state.  IDisposable
+ * pattern of Core/FFT.cs:49-76. */
+int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan);
+int p9idg_destroy(p9idg_plan* plan);
+int p9idg_set_kernel(p9idg_plan* plan, int32_t kernel);
+int p9idg_get_kernel(const p9idg_plan* plan, int32_t* kernel);
+/* Uploads uvw, frequencies and the partition to HBM (once per observation). */
+int p9idg_set_observation(p9idg_plan* plan, const double* uvw, int32_t nr_baselines,
+                          int32_t nr_timesteps, const double* freq, int32_t nr_channels,
+                          const p9idg_subgrid* meta, int64_t nr_subgrids);
+
+/* ---- Host-buffer drop-ins (synchronous) ---------------------------------- */
+/* IDG.Grid (IDG.cs:12-20): grid_out complex128[G][G] */
+int p9idg_grid(p9idg_plan* plan, const void* vis, int32_t vis_type, double* grid_out);
+/* IDG.DeGrid (IDG.cs:22-30): vis_out complex128[B][T][C] */
+int p9idg_degrid(p9idg_plan* plan, const double* grid, double* vis_out);
+/* IDG.GridW (IDG.cs:32-40): layers_out complex128[L][G][G] */
+int p9idg_gridw(p9idg_plan* plan, const void* vis, int32_t vis_type, double* layers_out);
+/* IDG.DeGridW (IDG.cs:42-50): grid complex128[G][G] (one grid, as the reference) */
+int p9idg_degridw(p9idg_plan* plan, const double* grid, double* vis_out);
+/* IDG.GridPSF (IDG.cs:52-66): flags uint8[B][T][C] (nonzero = flagged) or NULL */
+int p9idg_grid_psf(p9idg_plan* plan, const uint8_t* flags, double* grid_out);
+/* IDG.ToImage (IDG.cs:68-83): img_out double[G][G] */
+int p9idg_to_image(p9idg_plan* plan, const void* vis, int32_t vis_type, double* img_out);
+/* IDG.CalculatePSF (IDG.cs:85-114): psf_out double[G][G] */
+int p9idg_calculate_psf(p9idg_plan* plan, const uint8_t* flags, double* psf_out);
+/* GridW -> FFT.WStackIFFTFloat -> FFT.Shift (MajorCycle.cs:46-48, :88-90) */
+int p9idg_gridw_to_image(p9idg_plan* plan, const void* vis, int32_t vis_type, float* img_out);
+/* FFT.Shift(x); FFT.Forward(x); FFT.Shift(x); IDG.DeGridW (MajorCycle.cs:125-128) */
+int p9idg_image_to_vis(p9idg_plan* plan, const float* img, double* vis_out);
+
+/* ---- FFT.cs static steps (host buffers, computed on the GPU) ------------- */
+/* FFT.WStackIFFTFloat (FFT.cs:188-215): layers complex128[L][G][G] */
+int p9idg_wstack_ifft_float(p9idg_plan* plan, const double* layers, int32_t nr_layers,
+                            int64_t vis_count, float* out);
+/* FFT.Forward(float[,], norm) (FFT.cs:99-117): grid_out complex128[G][G] */
+int p9idg_fft_forward_float(p9idg_plan* plan, const float* img, double norm, double* grid_out);
+/* FFT.Backward(Complex[,], long) (FFT.cs:159-185) */
+int p9idg_fft_backward(p9idg_plan* plan, const double* grid, int64_t vis_count, double* out);
+/* FFT.Shift (FFT.cs:248-264), host, in place; n = rows = cols */
+int p9idg_shift_f32(float* a, int32_t n);
+int p9idg_shift_f64(double* a, int32_t n);
+int p9idg_shift_c128(double* a, int32_t n);
+/* Visibilities.Substract (Common.cs:305-318), host */
+int p9idg_substract(const double* vis0, const double* vis1, const uint8_t* flags, int64_t n,
+                    double* out);
+
+/* ---- Device-resident API (inputs stay in HBM) ---------------------------- */
+/* Copy visibilities into the plan's complex64[B][T][C] device buffer. */
+int p9idg_upload_vis(p9idg_plan* plan, const void* vis, int32_t vis_type);
+/* Fill the plan's visibility buffer with 1 (0 where flagged): PSF input,
+ * MajorCycle.cs:36-43. */
+int p9idg_set_psf_vis(p9idg_plan* plan, const uint8_t* flags);
+/* Grid the plan's visibilities (gridder -> subgrid FFT -> adder, one fused
+ * launch) into its grid buffer (complex64, layers for WSTACK). */
+int p9idg_exec_grid(p9idg_plan* plan, int32_t mode);
+/* Inverse-FFT the grid buffer(s) into the plan's float image
+ * (FFT.WStackIFFTFloat + Shift, or FFT.Backward + Shift). */
+int p9idg_exec_grid_to_image(p9idg_plan* plan, int32_t mode);
+/* ToUVW: image (float[G][G], device) -> model grid buffer (complex64). */
+int p9idg_exec_image_to_grid(p9idg_plan* plan);
+/* Degrid the model grid buffer (splitter -> subgrid FFT -> degridder, one
+ * fused launch) into the plan's complex64 model-visibility buffer. */
+int p9idg_exec_degrid(p9idg_plan* plan, int32_t mode);
+int p9idg_sync(p9idg_plan* plan);
+
+/* Device pointers of the plan's buffers (for zero-copy interop). */
+int p9idg_device_buffers(p9idg_plan* plan, void** vis, void** grid, void** model_grid,
+                         void** model_vis, void** image);
+/* Host copies of the plan's buffers. */
+int p9idg_download_grid(p9idg_plan* plan, int32_t mode, void* out_c64);
+int p9idg_upload_model_grid(p9idg_plan* plan, const void* grid_c64);
+int p9idg_download_model_vis(p9idg_plan* plan, void* out_c64);
+int p9idg_upload_image(p9idg_plan* plan, const float* img);
+int p9idg_download_image(p9idg_plan* plan, float* out);
+
+/* Milliseconds of the last exec_grid / exec_degrid kernel (HIP events on the
+ * plan's stream) and of the last FFT stage. */
+int p9idg_last_timings(p9idg_plan* plan, float* grid_ms, float* degrid_ms, float* fft_ms);
+/* Number of visibilities covered by the partition (sum of count * C). */
+int p9idg_gridded_visibilities(p9idg_plan* plan, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

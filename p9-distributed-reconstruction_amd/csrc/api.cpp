@@ -1,0 +1,833 @@
+// C ABI of libp9idg: the plan (handle) that owns device state, and the
+// entry points that mirror the reference's gridding API.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "p9idg_internal.h"
+
+namespace p9idg {
+
+const char* const kMsgPlace =
+    "could not place (all) visibilities on subgrid (too many channnels, kernel size too large)";
+const char* const kMsgSubgridRange = "subgrid falls not within grid";
+const char* const kMsgInvalidGrid = "invalid grid";
+
+namespace {
+thread_local std::string t_error;
+}
+void set_error(const std::string& msg) { t_error = msg; }
+const char* get_error() { return t_error.c_str(); }
+
+int fail(int status, const std::string& msg) {
+    set_error(msg);
+    return status;
+}
+int hip_fail(hipError_t e, const char* what) {
+    return fail(P9IDG_ERR_HIP, std::string("HIP error ") + hipGetErrorName(e) + " (" +
+                                   hipGetErrorString(e) + ") in " + what);
+}
+int fft_fail(hipfftResult r, const char* what) {
+    return fail(P9IDG_ERR_HIP, std::string("hipFFT error ") + std::to_string((int)r) + " in " + what);
+}
+
+}  // namespace p9idg
+
+using namespace p9idg;
+
+struct p9idg_plan {
+    p9idg_constants c{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t kernel = P9IDG_KERNEL_FP32;
+
+    // observation
+    bool have_obs = false;
+    int32_t B = 0, T = 0, C = 0;
+    int64_t nsub = 0;
+    bool plain_adder_ok = true;  // every subgrid passes Adder.cs:30-31
+    int64_t gridded_vis = 0;
+
+    // device buffers
+    double* d_uvw = nullptr;
+    double* d_wavenumbers = nullptr;
+    p9idg_subgrid* d_meta = nullptr;
+    int32_t* d_order = nullptr;
+    float2* d_vis = nullptr;
+    float2* d_model_vis = nullptr;
+    float2* d_grid = nullptr;
+    float2* d_layers = nullptr;
+    float2* d_model_grid = nullptr;
+    float* d_image = nullptr;
+    double* d_image64 = nullptr;
+    float2* d_twiddle = nullptr;
+    float2* d_phasor = nullptr;
+    uint8_t* d_flags = nullptr;
+    void* d_staging = nullptr;
+    size_t staging_bytes = 0;
+
+    hipfftHandle fft_one = 0, fft_layers = 0;
+    bool have_fft_one = false, have_fft_layers = false;
+
+    hipEvent_t ev[8] = {};
+    bool timed_grid = false, timed_degrid = false, timed_fft = false;
+};
+
+namespace {
+
+size_t padded(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+    if (*p) return P9IDG_OK;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, padded(count * sizeof(T)));
+    if (e != hipSuccess)
+        return fail(P9IDG_ERR_ALLOC, std::string("device allocation of ") +
+                                         std::to_string(count * sizeof(T)) +
+                                         " bytes failed: " + hipGetErrorString(e));
+    *p = static_cast<T*>(q);
+    return P9IDG_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int d) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != d) (void)hipSetDevice(d);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int check_plan(const p9idg_plan* p) {
+    if (!p) return fail(P9IDG_ERR_ARGUMENT, "plan is NULL");
+    return P9IDG_OK;
+}
+int check_obs(const p9idg_plan* p) {
+    if (int st = check_plan(p)) return st;
+    if (!p->have_obs) return fail(P9IDG_ERR_STATE, "no observation set (p9idg_set_observation)");
+    return P9IDG_OK;
+}
+
+size_t grid_elems(const p9idg_plan* p) { return (size_t)p->c.grid_size * p->c.grid_size; }
+size_t vis_elems(const p9idg_plan* p) { return (size_t)p->B * p->T * p->C; }
+
+int ensure_layers(p9idg_plan* p) {
+    return dev_alloc(&p->d_layers, grid_elems(p) * (size_t)p->c.wlayer_count);
+}
+
+int ensure_fft_one(p9idg_plan* p) {
+    if (p->have_fft_one) return P9IDG_OK;
+    P9_FFT(hipfftPlan2d(&p->fft_one, p->c.grid_size, p->c.grid_size, HIPFFT_C2C));
+    P9_FFT(hipfftSetStream(p->fft_one, p->stream));
+    p->have_fft_one = true;
+    return P9IDG_OK;
+}
+
+int ensure_fft_layers(p9idg_plan* p) {
+    if (p->have_fft_layers) return P9IDG_OK;
+    int n[2] = {p->c.grid_size, p->c.grid_size};
+    P9_FFT(hipfftPlanMany(&p->fft_layers, 2, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_C2C,
+                          p->c.wlayer_count));
+    P9_FFT(hipfftSetStream(p->fft_layers, p->stream));
+    p->have_fft_layers = true;
+    return P9IDG_OK;
+}
+
+int ensure_staging(p9idg_plan* p, size_t bytes) {
+    if (p->staging_bytes >= bytes) return P9IDG_OK;
+    if (p->d_staging) (void)hipFree(p->d_staging);
+    p->d_staging = nullptr;
+    p->staging_bytes = 0;
+    hipError_t e = hipMalloc(&p->d_staging, padded(bytes));
+    if (e != hipSuccess)
+        return fail(P9IDG_ERR_ALLOC, std::string("staging allocation failed: ") + hipGetErrorString(e));
+    p->staging_bytes = bytes;
+    return P9IDG_OK;
+}
+
+// complex128 host array -> complex64 device buffer
+int upload_c128(p9idg_plan* p, const double* src, float2* dst, size_t n) {
+    if (int st = ensure_staging(p, n * 16)) return st;
+    P9_HIP(hipMemcpyAsync(p->d_staging, src, n * 16, hipMemcpyHostToDevice, p->stream));
+    P9_HIP(launch_c128_to_c64(static_cast<const double2*>(p->d_staging), dst, n, p->stream));
+    return P9IDG_OK;
+}
+
+// complex64 device buffer -> complex128 host array
+int download_c128(p9idg_plan* p, const float2* src, double* dst, size_t n) {
+    std::vector<float2> tmp(n);
+    P9_HIP(hipMemcpyAsync(tmp.data(), src, n * sizeof(float2), hipMemcpyDeviceToHost, p->stream));
+    P9_HIP(hipStreamSynchronize(p->stream));
+    for (size_t i = 0; i < n; i++) {
+        dst[2 * i] = (double)tmp[i].x;
+        dst[2 * i + 1] = (double)tmp[i].y;
+    }
+    return P9IDG_OK;
+}
+
+KernelArgs make_args(const p9idg_plan* p, int mode) {
+    KernelArgs a{};
+    a.meta = p->d_meta;
+    a.order = p->d_order;
+    a.nr_subgrids = (int32_t)p->nsub;
+    a.uvw = p->d_uvw;
+    a.wavenumbers = p->d_wavenumbers;
+    a.vis = p->d_vis;
+    a.vis_out = p->d_model_vis;
+    a.twiddle = p->d_twiddle;
+    a.adder_phasor = p->d_phasor;
+    a.model_grid = p->d_model_grid;
+    a.G = p->c.grid_size;
+    a.T = p->T;
+    a.C = p->C;
+    a.L = p->c.wlayer_count;
+    a.image_size = p->c.scale_arcsec * (float)p->c.grid_size;  // GriddingConstants.cs:43
+    a.u_scale = 2 * M_PI / (double)a.image_size;                // Gridder.cs:38
+    a.w_offset_scale = 2 * M_PI * (double)p->c.wstep_lambda;    // Gridder.cs:40-41
+    const int S = p->c.subgrid_size;
+    a.degrid_scale = 1.0f / (float)(S * S);                      // Gridder.cs:151
+    a.mode = mode;
+    return a;
+}
+
+int do_exec_grid(p9idg_plan* p, int mode) {
+    if (int st = check_obs(p)) return st;
+    if (mode < P9IDG_GRID_PLAIN || mode > P9IDG_GRID_WSUM)
+        return fail(P9IDG_ERR_ARGUMENT, "unknown grid mode");
+    if (mode == P9IDG_GRID_PLAIN && !p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    DeviceGuard dg(p->device);
+    float2* target = p->d_grid;
+    size_t n = grid_elems(p);
+    if (mode == P9IDG_GRID_WSTACK) {
+        if (int st = ensure_layers(p)) return st;
+        target = p->d_layers;
+        n *= (size_t)p->c.wlayer_count;
+    }
+    KernelArgs a = make_args(p, mode);
+    a.grid = target;
+    P9_HIP(hipEventRecord(p->ev[0], p->stream));
+    P9_HIP(launch_fill_zero(target, n, p->stream));
+    P9_HIP(hipEventRecord(p->ev[1], p->stream));
+    hipError_t e;
+    if (p->kernel == P9IDG_KERNEL_FP32)
+        e = launch_grid_fp32(a, p->c.subgrid_size, p->stream);
+    else
+        e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, p->stream);
+    if (e != hipSuccess) return hip_fail(e, "gridder launch");
+    P9_HIP(hipEventRecord(p->ev[2], p->stream));
+    p->timed_grid = true;
+    return P9IDG_OK;
+}
+
+int do_exec_degrid(p9idg_plan* p, int mode) {
+    if (int st = check_obs(p)) return st;
+    if (mode != P9IDG_DEGRID_PLAIN && mode != P9IDG_DEGRID_WSTACK)
+        return fail(P9IDG_ERR_ARGUMENT, "unknown degrid mode");
+    DeviceGuard dg(p->device);
+    KernelArgs a = make_args(p, mode);
+    P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));  // Gridder.cs:96
+    P9_HIP(hipEventRecord(p->ev[3], p->stream));
+    hipError_t e;
+    if (p->kernel == P9IDG_KERNEL_FP32)
+        e = launch_degrid_fp32(a, p->c.subgrid_size, p->stream);
+    else
+        e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, p->stream);
+    if (e != hipSuccess) return hip_fail(e, "degridder launch");
+    P9_HIP(hipEventRecord(p->ev[4], p->stream));
+    p->timed_degrid = true;
+    return P9IDG_OK;
+}
+
+// Inverse FFT of the grid buffer(s) followed by the image extraction.
+int do_grid_to_image(p9idg_plan* p, int mode) {
+    if (int st = check_obs(p)) return st;
+    DeviceGuard dg(p->device);
+    const int G = p->c.grid_size;
+    const double inv_n = 1.0 / (double)p->c.visibilities_count;
+    P9_HIP(hipEventRecord(p->ev[5], p->stream));
+    if (mode == P9IDG_GRID_WSTACK) {
+        if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
+        if (int st = ensure_fft_layers(p)) return st;
+        P9_FFT(hipfftExecC2C(p->fft_layers, reinterpret_cast<hipfftComplex*>(p->d_layers),
+                             reinterpret_cast<hipfftComplex*>(p->d_layers), HIPFFT_BACKWARD));
+        P9_HIP(launch_layers_to_image(p->d_layers, p->c.wlayer_count, G, inv_n, 1, p->d_image,
+                                      p->stream));
+    } else {
+        if (int st = ensure_fft_one(p)) return st;
+        P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_grid),
+                             reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
+        P9_HIP(launch_layers_to_image(p->d_grid, 1, G, inv_n, 1, p->d_image, p->stream));
+    }
+    P9_HIP(hipEventRecord(p->ev[6], p->stream));
+    p->timed_fft = true;
+    return P9IDG_OK;
+}
+
+int do_image_to_grid(p9idg_plan* p) {
+    if (int st = check_obs(p)) return st;
+    DeviceGuard dg(p->device);
+    if (int st = ensure_fft_one(p)) return st;
+    P9_HIP(launch_image_to_grid(p->d_image, p->c.grid_size, 1, p->d_model_grid, p->stream));
+    P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_model_grid),
+                         reinterpret_cast<hipfftComplex*>(p->d_model_grid), HIPFFT_FORWARD));
+    return P9IDG_OK;
+}
+
+int upload_vis_any(p9idg_plan* p, const void* vis, int32_t type) {
+    if (int st = check_obs(p)) return st;
+    if (!vis) return fail(P9IDG_ERR_ARGUMENT, "visibilities are NULL");
+    DeviceGuard dg(p->device);
+    const size_t n = vis_elems(p);
+    if (type == P9IDG_C64) {
+        P9_HIP(hipMemcpyAsync(p->d_vis, vis, n * sizeof(float2), hipMemcpyHostToDevice, p->stream));
+    } else if (type == P9IDG_C128) {
+        if (int st = upload_c128(p, static_cast<const double*>(vis), p->d_vis, n)) return st;
+    } else {
+        return fail(P9IDG_ERR_ARGUMENT, "unknown visibility type");
+    }
+    return P9IDG_OK;
+}
+
+int set_psf_vis(p9idg_plan* p, const uint8_t* flags) {
+    if (int st = check_obs(p)) return st;
+    DeviceGuard dg(p->device);
+    const size_t n = vis_elems(p);
+    const uint8_t* dflags = nullptr;
+    if (flags) {
+        if (int st = dev_alloc(&p->d_flags, n)) return st;
+        P9_HIP(hipMemcpyAsync(p->d_flags, flags, n, hipMemcpyHostToDevice, p->stream));
+        dflags = p->d_flags;
+    }
+    P9_HIP(launch_psf_vis(dflags, p->d_vis, n, p->stream));
+    return P9IDG_OK;
+}
+
+int upload_grid_c128(p9idg_plan* p, const double* grid) {
+    if (!grid) return fail(P9IDG_ERR_ARGUMENT, "grid is NULL");
+    return upload_c128(p, grid, p->d_model_grid, grid_elems(p));
+}
+
+int sync(p9idg_plan* p) {
+    P9_HIP(hipStreamSynchronize(p->stream));
+    return P9IDG_OK;
+}
+
+template <typename T>
+void shift_host(T* a, int32_t n) {
+    const int n2 = n / 2;
+    for (int i = 0; i < n2; i++)
+        for (int j = 0; j < n2; j++) {
+            std::swap(a[(size_t)i * n + j], a[(size_t)(i + n2) * n + j + n2]);
+            std::swap(a[(size_t)(i + n2) * n + j], a[(size_t)i * n + j + n2]);
+        }
+}
+struct c128 {
+    double re, im;
+};
+
+}  // namespace
+
+extern "C" {
+
+int p9idg_api_version(void) { return P9IDG_API_VERSION; }
+const char* p9idg_last_error(void) { return get_error(); }
+
+int p9idg_partition(const p9idg_constants* c, const double* uvw, int32_t nr_baselines,
+                    int32_t nr_timesteps, const double* freq, int32_t nr_channels,
+                    p9idg_subgrid* out, int64_t capacity, int64_t* nr_subgrids,
+                    int64_t* baseline_offsets) {
+    if (!c || !freq || (!uvw && nr_baselines > 0 && nr_timesteps > 0))
+        return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    if (nr_baselines < 0 || nr_timesteps < 0 || nr_channels < 1)
+        return fail(P9IDG_ERR_ARGUMENT, "invalid dimensions");
+    std::vector<p9idg_subgrid> subs;
+    std::vector<int64_t> offs;
+    int st = partition_host(*c, uvw, nr_baselines, nr_timesteps, freq, nr_channels, subs, offs);
+    if (nr_subgrids) *nr_subgrids = (int64_t)subs.size();
+    if (st == P9IDG_ERR_PLACE) return fail(st, kMsgPlace);
+    if (st == P9IDG_ERR_SUBGRID_RANGE) return fail(st, kMsgSubgridRange);
+    if (st) return fail(st, "partition failed");
+    if (baseline_offsets) std::memcpy(baseline_offsets, offs.data(), offs.size() * sizeof(int64_t));
+    if (out) {
+        if ((int64_t)subs.size() > capacity)
+            return fail(P9IDG_ERR_CAPACITY, "output capacity " + std::to_string(capacity) +
+                                                " < " + std::to_string(subs.size()) + " subgrids");
+        std::memcpy(out, subs.data(), subs.size() * sizeof(p9idg_subgrid));
+    }
+    return P9IDG_OK;
+}
+
+int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
+    if (!c || !plan) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    *plan = nullptr;
+    if (c->grid_size <= 0 || (c->grid_size & 1))
+        return fail(P9IDG_ERR_UNSUPPORTED, "grid_size must be positive and even");
+    if (!subgrid_size_supported(c->subgrid_size))
+        return fail(P9IDG_ERR_UNSUPPORTED, "subgrid_size must be one of 8, 16, 24, 32, 48, 64");
+    if (c->subgrid_size >= c->grid_size) return fail(P9IDG_ERR_ARGUMENT, "subgrid_size >= grid_size");
+    if (c->wlayer_count < 1) return fail(P9IDG_ERR_ARGUMENT, "wlayer_count must be >= 1");
+    if (c->visibilities_count <= 0) return fail(P9IDG_ERR_ARGUMENT, "visibilities_count must be > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(P9IDG_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(P9IDG_ERR_ARGUMENT, "device index out of range");
+    auto* p = new p9idg_plan();
+    p->c = *c;
+    p->device = device;
+    DeviceGuard dg(device);
+    auto bail = [&](int st) {
+        p9idg_destroy(p);
+        return st;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    for (auto& ev : p->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+    }
+    const int S = c->subgrid_size, G = c->grid_size;
+    // tables: twiddles exp(+2 pi i m / S) and the adder phasor
+    // exp(i pi (x + y - S) / S) (Adder.cs:37-40, AdderWStack.cs:17-25), FP64 -> float
+    std::vector<float2> tw(S), ph((size_t)S * S);
+    for (int m = 0; m < S; m++) {
+        double a = 2.0 * M_PI * (double)m / (double)S;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    for (int y = 0; y < S; y++)
+        for (int x = 0; x < S; x++) {
+            double a = M_PI * (double)(x + y - S) / (double)S;
+            ph[(size_t)y * S + x] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+    int st;
+    if ((st = dev_alloc(&p->d_twiddle, (size_t)S))) return bail(st);
+    if ((st = dev_alloc(&p->d_phasor, (size_t)S * S))) return bail(st);
+    if ((st = dev_alloc(&p->d_grid, (size_t)G * G))) return bail(st);
+    if ((st = dev_alloc(&p->d_model_grid, (size_t)G * G))) return bail(st);
+    if ((st = dev_alloc(&p->d_image, (size_t)G * G))) return bail(st);
+    e = hipMemcpy(p->d_twiddle, tw.data(), S * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "twiddle upload"));
+    e = hipMemcpy(p->d_phasor, ph.data(), ph.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "phasor upload"));
+    *plan = p;
+    return P9IDG_OK;
+}
+
+int p9idg_destroy(p9idg_plan* p) {
+    if (!p) return P9IDG_OK;
+    DeviceGuard dg(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->have_fft_one) (void)hipfftDestroy(p->fft_one);
+    if (p->have_fft_layers) (void)hipfftDestroy(p->fft_layers);
+    dev_free(p->d_uvw);
+    dev_free(p->d_wavenumbers);
+    dev_free(p->d_meta);
+    dev_free(p->d_order);
+    dev_free(p->d_vis);
+    dev_free(p->d_model_vis);
+    dev_free(p->d_grid);
+    dev_free(p->d_layers);
+    dev_free(p->d_model_grid);
+    dev_free(p->d_image);
+    dev_free(p->d_image64);
+    dev_free(p->d_twiddle);
+    dev_free(p->d_phasor);
+    dev_free(p->d_flags);
+    if (p->d_staging) (void)hipFree(p->d_staging);
+    for (auto& ev : p->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return P9IDG_OK;
+}
+
+int p9idg_set_kernel(p9idg_plan* p, int32_t kernel) {
+    if (int st = check_plan(p)) return st;
+    if (kernel != P9IDG_KERNEL_FP32 && kernel != P9IDG_KERNEL_MFMA_BF16 &&
+        kernel != P9IDG_KERNEL_MFMA_FP16)
+        return fail(P9IDG_ERR_ARGUMENT, "unknown kernel");
+    p->kernel = kernel;
+    return P9IDG_OK;
+}
+
+int p9idg_get_kernel(const p9idg_plan* p, int32_t* kernel) {
+    if (int st = check_plan(p)) return st;
+    if (!kernel) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    *kernel = p->kernel;
+    return P9IDG_OK;
+}
+
+int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T,
+                          const double* freq, int32_t C, const p9idg_subgrid* meta,
+                          int64_t nsub) {
+    if (int st = check_plan(p)) return st;
+    if (B <= 0 || T <= 0 || C <= 0 || nsub < 0) return fail(P9IDG_ERR_ARGUMENT, "invalid dimensions");
+    if (!uvw || !freq || (!meta && nsub > 0)) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    if (nsub > INT32_MAX) return fail(P9IDG_ERR_ARGUMENT, "too many subgrids");
+    const int S = p->c.subgrid_size, G = p->c.grid_size, L = p->c.wlayer_count;
+    bool plain_ok = true;
+    int64_t gridded = 0;
+    for (int64_t i = 0; i < nsub; i++) {
+        const p9idg_subgrid& m = meta[i];
+        if (m.baseline_idx < 0 || m.baseline_idx >= B || m.time_sample_start < 0 ||
+            m.time_sample_count < 0 || (int64_t)m.time_sample_start + m.time_sample_count > T)
+            return fail(P9IDG_ERR_ARGUMENT, "subgrid " + std::to_string(i) + " indexes outside the observation");
+        if (m.w_lambda < -L || m.w_lambda >= L)
+            return fail(P9IDG_ERR_ARGUMENT, "subgrid " + std::to_string(i) + " has a w index outside the layers");
+        if (!(m.u_pixel >= 0 && m.u_pixel < G - S && m.v_pixel >= 0 && m.v_pixel < G - S))
+            plain_ok = false;
+        // AdderWStack addresses only 1 <= X,Y < G-S after mirroring; the
+        // splitter reads up to G - X - x.  Any other corner is skipped by
+        // both, so no further check is needed for memory safety.
+        gridded += (int64_t)m.time_sample_count * C;
+    }
+    DeviceGuard dg(p->device);
+    // drop the previous observation's buffers if the shape changed
+    if (p->have_obs && ((size_t)p->B * p->T * p->C != (size_t)B * T * C)) {
+        dev_free(p->d_vis);
+        dev_free(p->d_model_vis);
+        dev_free(p->d_flags);
+    }
+    if (p->have_obs && (size_t)p->B * p->T != (size_t)B * T) dev_free(p->d_uvw);
+    if (p->have_obs && p->C != C) dev_free(p->d_wavenumbers);
+    if (p->have_obs && p->nsub != nsub) {
+        dev_free(p->d_meta);
+        dev_free(p->d_order);
+    }
+    p->have_obs = false;
+    int st;
+    if ((st = dev_alloc(&p->d_uvw, (size_t)B * T * 3))) return st;
+    if ((st = dev_alloc(&p->d_wavenumbers, (size_t)C))) return st;
+    if ((st = dev_alloc(&p->d_meta, (size_t)std::max<int64_t>(nsub, 1)))) return st;
+    if ((st = dev_alloc(&p->d_order, (size_t)std::max<int64_t>(nsub, 1)))) return st;
+    if ((st = dev_alloc(&p->d_vis, (size_t)B * T * C))) return st;
+    if ((st = dev_alloc(&p->d_model_vis, (size_t)B * T * C))) return st;
+    // MathFunctions.cs:69-78: k = 2 pi f / c, (2.0f promoted to double)
+    std::vector<double> k(C);
+    for (int i = 0; i < C; i++) k[i] = ((2.0 * M_PI) * freq[i]) / 299792458.0;
+    // schedule: most visibilities first
+    std::vector<int32_t> order(nsub);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return meta[a].time_sample_count > meta[b].time_sample_count;
+    });
+    P9_HIP(hipMemcpyAsync(p->d_uvw, uvw, (size_t)B * T * 3 * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    P9_HIP(hipMemcpyAsync(p->d_wavenumbers, k.data(), C * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    if (nsub > 0) {
+        P9_HIP(hipMemcpyAsync(p->d_meta, meta, nsub * sizeof(p9idg_subgrid), hipMemcpyHostToDevice, p->stream));
+        P9_HIP(hipMemcpyAsync(p->d_order, order.data(), nsub * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+    }
+    P9_HIP(hipStreamSynchronize(p->stream));
+    p->B = B;
+    p->T = T;
+    p->C = C;
+    p->nsub = nsub;
+    p->plain_adder_ok = plain_ok;
+    p->gridded_vis = gridded;
+    p->have_obs = true;
+    return P9IDG_OK;
+}
+
+// ---- host-buffer drop-ins ----------------------------------------------------
+int p9idg_grid(p9idg_plan* p, const void* vis, int32_t type, double* grid_out) {
+    if (int st = check_obs(p)) return st;
+    if (!grid_out) return fail(P9IDG_ERR_ARGUMENT, "grid_out is NULL");
+    if (!p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    if (int st = upload_vis_any(p, vis, type)) return st;
+    if (int st = do_exec_grid(p, P9IDG_GRID_PLAIN)) return st;
+    return download_c128(p, p->d_grid, grid_out, grid_elems(p));
+}
+
+int p9idg_gridw(p9idg_plan* p, const void* vis, int32_t type, double* layers_out) {
+    if (int st = check_obs(p)) return st;
+    if (!layers_out) return fail(P9IDG_ERR_ARGUMENT, "layers_out is NULL");
+    if (int st = upload_vis_any(p, vis, type)) return st;
+    if (int st = do_exec_grid(p, P9IDG_GRID_WSTACK)) return st;
+    return download_c128(p, p->d_layers, layers_out, grid_elems(p) * p->c.wlayer_count);
+}
+
+static int degrid_host(p9idg_plan* p, const double* grid, double* vis_out, int mode) {
+    if (int st = check_obs(p)) return st;
+    if (!vis_out) return fail(P9IDG_ERR_ARGUMENT, "vis_out is NULL");
+    DeviceGuard dg(p->device);
+    if (int st = upload_grid_c128(p, grid)) return st;
+    if (int st = do_exec_degrid(p, mode)) return st;
+    return download_c128(p, p->d_model_vis, vis_out, vis_elems(p));
+}
+
+int p9idg_degrid(p9idg_plan* p, const double* grid, double* vis_out) {
+    return degrid_host(p, grid, vis_out, P9IDG_DEGRID_PLAIN);
+}
+
+int p9idg_degridw(p9idg_plan* p, const double* grid, double* vis_out) {
+    return degrid_host(p, grid, vis_out, P9IDG_DEGRID_WSTACK);
+}
+
+int p9idg_grid_psf(p9idg_plan* p, const uint8_t* flags, double* grid_out) {
+    if (int st = check_obs(p)) return st;
+    if (!grid_out) return fail(P9IDG_ERR_ARGUMENT, "grid_out is NULL");
+    if (!p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    if (int st = set_psf_vis(p, flags)) return st;
+    if (int st = do_exec_grid(p, P9IDG_GRID_PLAIN)) return st;
+    return download_c128(p, p->d_grid, grid_out, grid_elems(p));
+}
+
+static int image_f64_host(p9idg_plan* p, double* img_out) {
+    DeviceGuard dg(p->device);
+    if (int st = ensure_fft_one(p)) return st;
+    if (int st = dev_alloc(&p->d_image64, grid_elems(p))) return st;
+    P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_grid),
+                         reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
+    P9_HIP(launch_grid_to_image_f64(p->d_grid, p->c.grid_size,
+                                    1.0 / (double)p->c.visibilities_count, 1, p->d_image64,
+                                    p->stream));
+    P9_HIP(hipMemcpyAsync(img_out, p->d_image64, grid_elems(p) * sizeof(double),
+                          hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
+int p9idg_to_image(p9idg_plan* p, const void* vis, int32_t type, double* img_out) {
+    if (int st = check_obs(p)) return st;
+    if (!img_out) return fail(P9IDG_ERR_ARGUMENT, "img_out is NULL");
+    if (!p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    if (int st = upload_vis_any(p, vis, type)) return st;
+    if (int st = do_exec_grid(p, P9IDG_GRID_PLAIN)) return st;
+    return image_f64_host(p, img_out);
+}
+
+int p9idg_calculate_psf(p9idg_plan* p, const uint8_t* flags, double* psf_out) {
+    if (int st = check_obs(p)) return st;
+    if (!psf_out) return fail(P9IDG_ERR_ARGUMENT, "psf_out is NULL");
+    if (!p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    if (int st = set_psf_vis(p, flags)) return st;
+    if (int st = do_exec_grid(p, P9IDG_GRID_PLAIN)) return st;
+    return image_f64_host(p, psf_out);
+}
+
+int p9idg_gridw_to_image(p9idg_plan* p, const void* vis, int32_t type, float* img_out) {
+    if (int st = check_obs(p)) return st;
+    if (!img_out) return fail(P9IDG_ERR_ARGUMENT, "img_out is NULL");
+    if (int st = upload_vis_any(p, vis, type)) return st;
+    if (int st = do_exec_grid(p, P9IDG_GRID_WSTACK)) return st;
+    if (int st = do_grid_to_image(p, P9IDG_GRID_WSTACK)) return st;
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(img_out, p->d_image, grid_elems(p) * sizeof(float),
+                          hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
+int p9idg_image_to_vis(p9idg_plan* p, const float* img, double* vis_out) {
+    if (int st = check_obs(p)) return st;
+    if (!img || !vis_out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(p->d_image, img, grid_elems(p) * sizeof(float), hipMemcpyHostToDevice,
+                          p->stream));
+    if (int st = do_image_to_grid(p)) return st;
+    if (int st = do_exec_degrid(p, P9IDG_DEGRID_WSTACK)) return st;
+    return download_c128(p, p->d_model_vis, vis_out, vis_elems(p));
+}
+
+// ---- FFT.cs static steps --------------------------------------------------------
+int p9idg_wstack_ifft_float(p9idg_plan* p, const double* layers, int32_t L, int64_t vis_count,
+                            float* out) {
+    if (int st = check_plan(p)) return st;
+    if (!layers || !out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    if (L != p->c.wlayer_count) return fail(P9IDG_ERR_ARGUMENT, "layer count differs from the plan's");
+    if (vis_count == 0) return fail(P9IDG_ERR_ARGUMENT, "vis_count is 0");
+    DeviceGuard dg(p->device);
+    if (int st = ensure_layers(p)) return st;
+    if (int st = ensure_fft_layers(p)) return st;
+    const size_t n = grid_elems(p) * L;
+    if (int st = upload_c128(p, layers, p->d_layers, n)) return st;
+    P9_FFT(hipfftExecC2C(p->fft_layers, reinterpret_cast<hipfftComplex*>(p->d_layers),
+                         reinterpret_cast<hipfftComplex*>(p->d_layers), HIPFFT_BACKWARD));
+    P9_HIP(launch_layers_to_image(p->d_layers, L, p->c.grid_size, 1.0 / (double)vis_count, 0,
+                                  p->d_image, p->stream));
+    P9_HIP(hipMemcpyAsync(out, p->d_image, grid_elems(p) * sizeof(float), hipMemcpyDeviceToHost,
+                          p->stream));
+    return sync(p);
+}
+
+int p9idg_fft_forward_float(p9idg_plan* p, const float* img, double norm, double* grid_out) {
+    if (int st = check_plan(p)) return st;
+    if (!img || !grid_out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    if (int st = ensure_fft_one(p)) return st;
+    const int G = p->c.grid_size;
+    P9_HIP(hipMemcpyAsync(p->d_image, img, grid_elems(p) * sizeof(float), hipMemcpyHostToDevice,
+                          p->stream));
+    P9_HIP(launch_image_to_grid(p->d_image, G, 0, p->d_model_grid, p->stream));
+    P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_model_grid),
+                         reinterpret_cast<hipfftComplex*>(p->d_model_grid), HIPFFT_FORWARD));
+    if (norm != 1.0) P9_HIP(launch_scale_c64(p->d_model_grid, grid_elems(p), 1.0 / norm, p->stream));
+    return download_c128(p, p->d_model_grid, grid_out, grid_elems(p));
+}
+
+int p9idg_fft_backward(p9idg_plan* p, const double* grid, int64_t vis_count, double* out) {
+    if (int st = check_plan(p)) return st;
+    if (!grid || !out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    if (vis_count == 0) return fail(P9IDG_ERR_ARGUMENT, "vis_count is 0");
+    DeviceGuard dg(p->device);
+    if (int st = ensure_fft_one(p)) return st;
+    if (int st = dev_alloc(&p->d_image64, grid_elems(p))) return st;
+    if (int st = upload_c128(p, grid, p->d_grid, grid_elems(p))) return st;
+    P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_grid),
+                         reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
+    P9_HIP(launch_grid_to_image_f64(p->d_grid, p->c.grid_size, 1.0 / (double)vis_count, 0,
+                                    p->d_image64, p->stream));
+    P9_HIP(hipMemcpyAsync(out, p->d_image64, grid_elems(p) * sizeof(double),
+                          hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
+int p9idg_shift_f32(float* a, int32_t n) {
+    if (!a || n < 0) return fail(P9IDG_ERR_ARGUMENT, "invalid argument");
+    shift_host(a, n);
+    return P9IDG_OK;
+}
+int p9idg_shift_f64(double* a, int32_t n) {
+    if (!a || n < 0) return fail(P9IDG_ERR_ARGUMENT, "invalid argument");
+    shift_host(a, n);
+    return P9IDG_OK;
+}
+int p9idg_shift_c128(double* a, int32_t n) {
+    if (!a || n < 0) return fail(P9IDG_ERR_ARGUMENT, "invalid argument");
+    shift_host(reinterpret_cast<c128*>(a), n);
+    return P9IDG_OK;
+}
+
+int p9idg_substract(const double* vis0, const double* vis1, const uint8_t* flags, int64_t n,
+                    double* out) {
+    if (!vis0 || !vis1 || !out || n < 0) return fail(P9IDG_ERR_ARGUMENT, "invalid argument");
+    for (int64_t i = 0; i < n; i++) {
+        if (flags && flags[i]) {
+            out[2 * i] = 0.0;
+            out[2 * i + 1] = 0.0;
+        } else {
+            out[2 * i] = vis0[2 * i] - vis1[2 * i];
+            out[2 * i + 1] = vis0[2 * i + 1] - vis1[2 * i + 1];
+        }
+    }
+    return P9IDG_OK;
+}
+
+// ---- device-resident API -----------------------------------------------------------
+int p9idg_upload_vis(p9idg_plan* p, const void* vis, int32_t type) {
+    if (int st = upload_vis_any(p, vis, type)) return st;
+    return sync(p);
+}
+int p9idg_set_psf_vis(p9idg_plan* p, const uint8_t* flags) {
+    if (int st = set_psf_vis(p, flags)) return st;
+    return sync(p);
+}
+int p9idg_exec_grid(p9idg_plan* p, int32_t mode) { return do_exec_grid(p, mode); }
+int p9idg_exec_grid_to_image(p9idg_plan* p, int32_t mode) { return do_grid_to_image(p, mode); }
+int p9idg_exec_image_to_grid(p9idg_plan* p) { return do_image_to_grid(p); }
+int p9idg_exec_degrid(p9idg_plan* p, int32_t mode) { return do_exec_degrid(p, mode); }
+int p9idg_sync(p9idg_plan* p) {
+    if (int st = check_plan(p)) return st;
+    DeviceGuard dg(p->device);
+    return sync(p);
+}
+
+int p9idg_device_buffers(p9idg_plan* p, void** vis, void** grid, void** model_grid,
+                         void** model_vis, void** image) {
+    if (int st = check_plan(p)) return st;
+    if (vis) *vis = p->d_vis;
+    if (grid) *grid = p->d_grid;
+    if (model_grid) *model_grid = p->d_model_grid;
+    if (model_vis) *model_vis = p->d_model_vis;
+    if (image) *image = p->d_image;
+    return P9IDG_OK;
+}
+
+int p9idg_download_grid(p9idg_plan* p, int32_t mode, void* out) {
+    if (int st = check_obs(p)) return st;
+    if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    const float2* src = p->d_grid;
+    size_t n = grid_elems(p);
+    if (mode == P9IDG_GRID_WSTACK) {
+        if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
+        src = p->d_layers;
+        n *= p->c.wlayer_count;
+    }
+    P9_HIP(hipMemcpyAsync(out, src, n * sizeof(float2), hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
+int p9idg_upload_model_grid(p9idg_plan* p, const void* grid) {
+    if (int st = check_plan(p)) return st;
+    if (!grid) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(p->d_model_grid, grid, grid_elems(p) * sizeof(float2),
+                          hipMemcpyHostToDevice, p->stream));
+    return sync(p);
+}
+
+int p9idg_download_model_vis(p9idg_plan* p, void* out) {
+    if (int st = check_obs(p)) return st;
+    if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(out, p->d_model_vis, vis_elems(p) * sizeof(float2),
+                          hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
+int p9idg_upload_image(p9idg_plan* p, const float* img) {
+    if (int st = check_plan(p)) return st;
+    if (!img) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(p->d_image, img, grid_elems(p) * sizeof(float), hipMemcpyHostToDevice,
+                          p->stream));
+    return sync(p);
+}
+
+int p9idg_download_image(p9idg_plan* p, float* out) {
+    if (int st = check_plan(p)) return st;
+    if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(out, p->d_image, grid_elems(p) * sizeof(float), hipMemcpyDeviceToHost,
+                          p->stream));
+    return sync(p);
+}
+
+int p9idg_last_timings(p9idg_plan* p, float* grid_ms, float* degrid_ms, float* fft_ms) {
+    if (int st = check_plan(p)) return st;
+    DeviceGuard dg(p->device);
+    P9_HIP(hipStreamSynchronize(p->stream));
+    float ms = 0.f;
+    if (grid_ms) {
+        *grid_ms = -1.f;
+        if (p->timed_grid && hipEventElapsedTime(&ms, p->ev[1], p->ev[2]) == hipSuccess) *grid_ms = ms;
+    }
+    if (degrid_ms) {
+        *degrid_ms = -1.f;
+        if (p->timed_degrid && hipEventElapsedTime(&ms, p->ev[3], p->ev[4]) == hipSuccess) *degrid_ms = ms;
+    }
+    if (fft_ms) {
+        *fft_ms = -1.f;
+        if (p->timed_fft && hipEventElapsedTime(&ms, p->ev[5], p->ev[6]) == hipSuccess) *fft_ms = ms;
+    }
+    return P9IDG_OK;
+}
+
+int p9idg_gridded_visibilities(p9idg_plan* p, int64_t* n) {
+    if (int st = check_obs(p)) return st;
+    if (!n) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    *n = p->gridded_vis;
+    return P9IDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Element-wise HBM-bound kernels around the gridder: buffer fills, visibility
+// conversion, the PSF visibilities, and the image <-> grid steps around the
+// G x G inverse / forward FFTs (FFT.cs:99-117, :159-215, :248-264).
+#include "device_common.h"
+
+namespace p9idg {
+namespace dev {
+
+__global__ void fill_zero_kernel(float4* p, size_t n4) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n4; i += stride) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void c128_to_c64_kernel(const double2* in, float2* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const double2 v = in[i];
+        out[i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// MajorCycle.cs:36-43 / IDG.cs:54-59: 1 where unflagged, 0 where flagged.
+__global__ void psf_vis_kernel(const uint8_t* flags, float2* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = make_float2((flags && flags[i]) ? 0.f : 1.f, 0.f);
+}
+
+// FFT.WStackIFFTFloat (FFT.cs:194-211) + FFT.Shift (FFT.cs:248-264): the
+// layers already hold their inverse FFTs; sum (float)(Re / N) over layers in
+// layer order and write the quadrant-swapped image.
+__global__ void layers_to_image_kernel(const float2* layers, int L, int G, double inv_n,
+                                       int shift, float* img) {
+    const size_t GG = (size_t)G * G;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < GG; i += stride) {
+        const int y = (int)(i / G), x = (int)(i - (size_t)y * G);
+        const size_t src = shift ? (size_t)shifted(y, G) * G + shifted(x, G) : i;
+        float acc = 0.f;
+        for (int k = 0; k < L; k++) acc += (float)((double)layers[k * GG + src].x * inv_n);
+        img[i] = acc;
+    }
+}
+
+// FFT.Backward(Complex[,], long) (FFT.cs:159-185) + Shift, double output.
+__global__ void grid_to_image_f64_kernel(const float2* grid, int G, double inv_n, int shift,
+                                         double* img) {
+    const size_t GG = (size_t)G * G;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < GG; i += stride) {
+        const int y = (int)(i / G), x = (int)(i - (size_t)y * G);
+        const size_t src = shift ? (size_t)shifted(y, G) * G + shifted(x, G) : i;
+        img[i] = (double)grid[src].x * inv_n;
+    }
+}
+
+// MajorCycle.cs:125-126: FFT.Shift(x) then the real->complex copy of
+// FFT.Forward(float[,]) (FFT.cs:105-107); the forward FFT follows.
+__global__ void image_to_grid_kernel(const float* img, int G, int shift, float2* grid) {
+    const size_t GG = (size_t)G * G;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < GG; i += stride) {
+        const int y = (int)(i / G), x = (int)(i - (size_t)y * G);
+        const size_t src = shift ? (size_t)shifted(y, G) * G + shifted(x, G) : i;
+        grid[i] = make_float2(img[src], 0.f);
+    }
+}
+
+__global__ void scale_c64_kernel(float2* p, size_t n, double inv) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        float2 v = p[i];
+        p[i] = make_float2((float)((double)v.x * inv), (float)((double)v.y * inv));
+    }
+}
+
+inline unsigned blocks_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
+}
+
+}  // namespace dev
+
+hipError_t launch_fill_zero(float2* p, size_t n, hipStream_t st) {
+    // n complex64 values; buffers are allocated in multiples of 2 complex.
+    const size_t n4 = (n + 1) / 2;
+    hipLaunchKernelGGL(dev::fill_zero_kernel, dim3(dev::blocks_for(n4)), dim3(256), 0, st,
+                       reinterpret_cast<float4*>(p), n4);
+    return hipGetLastError();
+}
+hipError_t launch_c128_to_c64(const double2* in, float2* out, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(dev::c128_to_c64_kernel, dim3(dev::blocks_for(n)), dim3(256), 0, st, in,
+                       out, n);
+    return hipGetLastError();
+}
+hipError_t launch_psf_vis(const uint8_t* flags, float2* out, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(dev::psf_vis_kernel, dim3(dev::blocks_for(n)), dim3(256), 0, st, flags,
+                       out, n);
+    return hipGetLastError();
+}
+hipError_t launch_layers_to_image(const float2* layers, int L, int G, double inv_n, int shift,
+                                  float* img, hipStream_t st) {
+    hipLaunchKernelGGL(dev::layers_to_image_kernel, dim3(dev::blocks_for((size_t)G * G)),
+                       dim3(256), 0, st, layers, L, G, inv_n, shift, img);
+    return hipGetLastError();
+}
+hipError_t launch_grid_to_image_f64(const float2* grid, int G, double inv_n, int shift,
+                                    double* img, hipStream_t st) {
+    hipLaunchKernelGGL(dev::grid_to_image_f64_kernel, dim3(dev::blocks_for((size_t)G * G)),
+                       dim3(256), 0, st, grid, G, inv_n, shift, img);
+    return hipGetLastError();
+}
+hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(dev::image_to_grid_kernel, dim3(dev::blocks_for((size_t)G * G)),
+                       dim3(256), 0, st, img, G, shift, grid);
+    return hipGetLastError();
+}
+hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st) {
+    hipLaunchKernelGGL(dev::scale_c64_kernel, dim3(dev::blocks_for(n)), dim3(256), 0, st, p, n,
+                       inv_norm);
+    return hipGetLastError();
+}
+
+}  // namespace p9idg

@@ -1,0 +1,82 @@
+// Internal declarations shared by the host sources of libp9idg.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "p9idg.h"
+
+namespace p9idg {
+
+// Thread-local error text returned by p9idg_last_error().
+void set_error(const std::string& msg);
+const char* get_error();
+
+// Messages of the reference's exceptions (Partitioner.cs:86,91,118; Adder.cs:49).
+extern const char* const kMsgPlace;
+extern const char* const kMsgSubgridRange;
+extern const char* const kMsgInvalidGrid;
+
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int fft_fail(hipfftResult r, const char* what);
+
+#define P9_HIP(call)                                             \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::p9idg::hip_fail(e_, #call); \
+    } while (0)
+#define P9_FFT(call)                                             \
+    do {                                                         \
+        hipfftResult r_ = (call);                                \
+        if (r_ != HIPFFT_SUCCESS) return ::p9idg::fft_fail(r_, #call); \
+    } while (0)
+
+// Host partitioner (partition.cpp).
+int partition_host(const p9idg_constants& c, const double* uvw, int B, int T, const double* freq,
+                   int C, std::vector<p9idg_subgrid>& out, std::vector<int64_t>& bl_offsets);
+
+bool subgrid_size_supported(int s);
+
+// Kernel launch parameters shared by host and device code.
+struct KernelArgs {
+    const p9idg_subgrid* meta;
+    const int32_t* order;  // subgrid schedule (longest first)
+    int32_t nr_subgrids;
+    const double* uvw;     // [B][T][3]
+    const double* wavenumbers;  // [C]
+    const float2* vis;     // [B][T][C]
+    float2* vis_out;       // [B][T][C] (degrid)
+    const float2* twiddle; // [S] exp(+2 pi i m / S)
+    const float2* adder_phasor;  // [S][S] exp(i pi (x + y - S) / S)
+    float2* grid;          // grid or layer stack (gridder output)
+    const float2* model_grid;  // [G][G] (degrid input, unshifted as the caller passes it)
+    int32_t G, T, C, L;
+    float image_size;
+    double u_scale;        // 2 pi / image_size  (Gridder.cs:40)
+    double w_offset_scale; // 2 pi * WStepLambda (0 in the reference)
+    float degrid_scale;    // 1.0f / (S * S)  (Gridder.cs:151)
+    int32_t mode;
+};
+
+// Launchers (kernels_*.hip); S is the subgrid size.
+hipError_t launch_grid_fp32(const KernelArgs& a, int S, hipStream_t st);
+hipError_t launch_degrid_fp32(const KernelArgs& a, int S, hipStream_t st);
+hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st);
+hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st);
+hipError_t launch_fill_zero(float2* p, size_t n, hipStream_t st);
+hipError_t launch_c128_to_c64(const double2* in, float2* out, size_t n, hipStream_t st);
+hipError_t launch_psf_vis(const uint8_t* flags, float2* out, size_t n, hipStream_t st);
+hipError_t launch_layers_to_image(const float2* layers, int L, int G, double inv_n, int shift,
+                                  float* img, hipStream_t st);
+hipError_t launch_grid_to_image_f64(const float2* grid, int G, double inv_n, int shift,
+                                    double* img, hipStream_t st);
+hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid,
+                                hipStream_t st);
+hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st);
+
+}  // namespace p9idg

@@ -1,0 +1,24 @@
+"""MI355X (gfx950) image-domain gridder with w-stacking.
+
+The gridding API of the reference (Core/ImageDomainGridder, Core/FFT.cs)
+backed by the HIP library libp9idg.so; see p9idg.idg.
+"""
+from .idg import (  # noqa: F401
+    FFT,
+    IDG,
+    GriddingConstants,
+    P9IDGError,
+    Partition,
+    Partitioner,
+    Plan,
+    SUBGRID_DTYPE,
+    Visibilities,
+)
+from . import _lib  # noqa: F401
+from ._lib import (  # noqa: F401
+    DEGRID_PLAIN,
+    DEGRID_WSTACK,
+    GRID_PLAIN,
+    GRID_WSTACK,
+    GRID_WSUM,
+)
