@@ -1,0 +1,261 @@
+#!/usr/bin/env python3
+"""Grid + degrid throughput of the image-domain gridder on one MI355X.
+
+Workload: the "large" observation (3072^2 image, 32 w-layers, 64 antennas =
+2016 baselines x 2000 timesteps x 8 channels, subgrid 32, kernel 8), inputs
+resident in HBM.  One step = IDG.GridW (zero the 32 layers, fused gridder ->
+subgrid FFT -> w-stack adder) + IDG.DeGridW (fused splitter -> subgrid FFT ->
+degridder) over every gridded visibility.  Visibilities per second count the
+visibilities the partition places on subgrids; the total is reported beside.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the FP64
+oracle (a C++ restatement of the reference's Parallel.For path) timed on the
+host cores on a stride sample of the subgrids; the same sample gives the
+error of the GPU result against the oracle.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "p9-distributed-reconstruction_amd"))
+
+METRIC = "Mvis/s grid+degrid, 3072² 32 w-layers: split-BF16 MFMA vs FP32, ≤1e-5 err"
+PEAK_TFLOPS = {"fp32": 157.3, "mfma_bf16": 2500.0, "mfma_fp16": 2500.0}
+DTYPE = {"fp32": "f32", "mfma_bf16": "bf16 hi/lo split, f32 accumulate", "mfma_fp16": "f16 hi/lo split, f32 accumulate"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return max(1, min(n, 16))
+
+
+def model_image(ob):
+    """Model image with the sky's point sources at their nearest pixels."""
+    G = ob.c.GridSize
+    cell = float(ob.c.ScaleArcSec)
+    img = np.zeros((G, G), dtype=np.float32)
+    for flux, l, m, sigma in ob.sources:
+        x = G // 2 + int(round(l / cell))
+        y = G // 2 + int(round(m / cell))
+        if 0 <= x < G and 0 <= y < G:
+            img[y, x] += flux
+    return img
+
+
+def sample_subgrids(meta, C, target_vis):
+    """Every k-th subgrid, k chosen so the sample holds about target_vis visibilities."""
+    total = int(meta["timeSampleCount"].sum()) * C
+    k = max(1, int(round(total / max(target_vis, 1))))
+    return np.ascontiguousarray(meta[::k])
+
+
+def cpu_baseline_and_error(args, ob, part, model_grid_c64, kernel):
+    """Times the FP64 oracle on a bounded sample and compares the GPU on it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import p9idg
+
+    threads = host_threads()
+    O.set_threads(threads)
+    c = ob.c
+    oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize,
+                          c.MaxTimestepsPerSubgrid, float(c.ScaleArcSec), c.WLayerCount, c.WStep)
+    C = len(ob.freq)
+    meta = part.subgrids
+    vis128 = ob.vis.astype(np.complex128)
+    model128 = model_grid_c64.astype(np.complex128)
+    # calibrate the oracle's rate on a small sample, then size the real one
+    probe = sample_subgrids(meta, C, 20000)
+    t = time.perf_counter()
+    sub = O.gridder_forward(oc, probe, ob.uvw, vis128, ob.freq)
+    O.gridder_backward(oc, probe, sub, ob.uvw, ob.freq)
+    rate = int(probe["timeSampleCount"].sum()) * C / max(time.perf_counter() - t, 1e-6)
+    target = max(20000, int(rate * args.cpu_seconds))
+    sample = sample_subgrids(meta, C, target)
+    n_vis = int(sample["timeSampleCount"].sum()) * C
+    log("cpu baseline: %d subgrids, %d visibilities, %d threads" % (len(sample), n_vis, threads))
+
+    t0 = time.perf_counter()
+    sub = O.gridder_forward(oc, sample, ob.uvw, vis128, ob.freq)
+    sub = O.subgrid_fft_forward(oc, sub)
+    layers_ref = O.adderw_add(oc, sample, sub)
+    t1 = time.perf_counter()
+    g = np.array(model128, copy=True)
+    O.shift(g)  # IDG.DeGridW shifts the grid first (IDG.cs:44)
+    sub = O.adderw_split(oc, sample, g)
+    sub = O.subgrid_fft_backward(oc, sub)
+    vis_ref = O.gridder_backward(oc, sample, sub, ob.uvw, ob.freq)
+    t2 = time.perf_counter()
+    del sub, g
+    for k in range(layers_ref.shape[0]):
+        O.shift(layers_ref[k])  # IDG.GridW shifts each layer (IDG.cs:37)
+    cpu = {
+        "value": n_vis / (t2 - t0) / 1e6,
+        "unit": "Mvis/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "every %d-th subgrid of the workload: %d subgrids, %d visibilities; FP64 grid %.1f s + degrid %.1f s"
+        % (max(1, len(meta) // max(len(sample), 1)), len(sample), n_vis, t1 - t0, t2 - t1),
+    }
+
+    # GPU on the same sample
+    with p9idg.Plan(c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, sample)
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        layers = plan.download_grid(p9idg.GRID_WSTACK)
+        plan.upload_model_grid(model_grid_c64)
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        vis = plan.download_model_vis()
+    err_grid = float(np.abs(layers - layers_ref).max() / np.abs(layers_ref).max())
+    covered = np.zeros(vis.shape[:2], dtype=bool)
+    for s in sample:
+        covered[s["baselineIdx"], s["timeSampleStart"]:s["timeSampleStart"] + s["timeSampleCount"]] = True
+    dv = np.abs(vis - vis_ref)[covered]
+    err_vis = float(dv.max() / np.abs(vis_ref[covered]).max())
+    rms_vis = float(np.sqrt(np.mean(dv ** 2)) / np.sqrt(np.mean(np.abs(vis_ref[covered]) ** 2)))
+    err = {
+        "sample_subgrids": int(len(sample)),
+        "gridw_layers_max_rel": err_grid,
+        "degridw_vis_max_rel": err_vis,
+        "degridw_vis_rms_rel": rms_vis,
+    }
+    return cpu, err
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kernel", default="fp32", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    import p9idg
+    from p9idg import observation
+
+    t = time.perf_counter()
+    ob = observation.large(threads=host_threads())
+    log("workload generated in %.1f s" % (time.perf_counter() - t))
+    t = time.perf_counter()
+    part = ob.partition()
+    t_part = time.perf_counter() - t
+    c = ob.c
+    S = c.SubgridSize
+    log("partition: %d subgrids in %.3f s" % (part.total, t_part))
+
+    plan = p9idg.Plan(c, device=0, kernel=args.kernel)
+    plan.set_observation(ob.uvw, ob.freq, part)
+    plan.upload_vis(ob.vis)
+    plan.upload_image(model_image(ob))
+    plan.exec_image_to_grid()  # ToUVW of the model (MajorCycle.cs:125-127)
+    plan.sync()
+    n_vis = plan.gridded_visibilities
+    log("gridded visibilities: %d of %d" % (n_vis, ob.vis.size))
+
+    def step():
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+
+    for _ in range(args.warmup):
+        step()
+    plan.sync()
+    log("warmup done")
+    plan.timing_reset(True)
+    plan.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    plan.sync()
+    t1 = time.perf_counter()
+    g_ms, g_n, d_ms, d_n = plan.timing_totals()
+    plan.timing_reset(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n_vis / (ms_per_step / 1e3) / 1e6
+    avg_g = g_ms / max(g_n, 1)
+    avg_d = d_ms / max(d_n, 1)
+    log("step %.3f ms: gridder %.3f ms, degridder %.3f ms" % (ms_per_step, avg_g, avg_d))
+
+    # layer FFT + image extraction, timed once (not part of the metric)
+    t = time.perf_counter()
+    plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+    plan.sync()
+    fft_wall = (time.perf_counter() - t) * 1e3
+
+    flops_per_launch = 8.0 * S * S * n_vis  # complex MAC per pixel per visibility
+    dom_name, dom_ms = ("gridder", avg_g) if avg_g >= avg_d else ("degridder", avg_d)
+    achieved = flops_per_launch / (dom_ms / 1e3) / 1e12
+    peak = PEAK_TFLOPS[args.kernel]
+    roofline = {
+        "bound": "mfma",
+        "kernel": dom_name,
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": achieved / peak,
+        "traffic": None,
+    }
+    cpu, err = None, None
+    if not args.no_cpu_baseline:
+        model_grid = plan.download_model_grid()
+        plan.close()
+        cpu, err = cpu_baseline_and_error(args, ob, part, model_grid, args.kernel)
+    else:
+        plan.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.kernel],
+        "data": "synthetic (seeded; see p9idg/observation.py 'large')",
+        "config": {
+            "workload": "large: 3072^2 image, 32 w-layers, 2016 baselines x 2000 timesteps x 8 channels, subgrid 32, kernel 8",
+            "kernel": args.kernel,
+            "gridded_visibilities": int(n_vis),
+            "total_visibilities": int(ob.vis.size),
+            "subgrids": int(part.total),
+        },
+        "stages": {
+            "gridder_ms": avg_g,
+            "degridder_ms": avg_d,
+            "layer_ifft_and_image_ms": fft_wall,
+            "partition_ms": t_part * 1e3,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "error_vs_oracle": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

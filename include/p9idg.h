@@ -198,6 +198,7 @@ int p9idg_device_buffers(p9idg_plan* plan, void** vis, void** grid, void** model
 /* Host copies of the plan's buffers. */
 int p9idg_download_grid(p9idg_plan* plan, int32_t mode, void* out_c64);
 int p9idg_upload_model_grid(p9idg_plan* plan, const void* grid_c64);
+int p9idg_download_model_grid(p9idg_plan* plan, void* out_c64);
 int p9idg_download_model_vis(p9idg_plan* plan, void* out_c64);
 int p9idg_upload_image(p9idg_plan* plan, const float* img);
 int p9idg_download_image(p9idg_plan* plan, float* out);
@@ -205,6 +206,12 @@ int p9idg_download_image(p9idg_plan* plan, float* out);
 /* Milliseconds of the last exec_grid / exec_degrid kernel (HIP events on the
  * plan's stream) and of the last FFT stage. */
 int p9idg_last_timings(p9idg_plan* plan, float* grid_ms, float* degrid_ms, float* fft_ms);
+/* Collect HIP-event spans around every gridder / degridder kernel launch
+ * (collect != 0) from now on, dropping earlier ones; totals are summed over
+ * the collected spans. */
+int p9idg_timing_reset(p9idg_plan* plan, int32_t collect);
+int p9idg_timing_totals(p9idg_plan* plan, double* grid_ms, int64_t* grid_launches,
+                        double* degrid_ms, int64_t* degrid_launches);
 /* Number of visibilities covered by the partition (sum of count * C). */
 int p9idg_gridded_visibilities(p9idg_plan* plan, int64_t* n);
 

@@ -74,6 +74,16 @@ struct p9idg_plan {
 
     hipEvent_t ev[8] = {};
     bool timed_grid = false, timed_degrid = false, timed_fft = false;
+
+    // Event pairs of every gridder / degridder launch since the last reset
+    // (p9idg_timing_reset), for averages over a timed region.
+    struct Span {
+        hipEvent_t start, stop;
+        int kind;  // 0 grid, 1 degrid
+    };
+    std::vector<hipEvent_t> event_pool;
+    std::vector<Span> spans;
+    bool collect = false;
 };
 
 namespace {
@@ -177,6 +187,31 @@ int download_c128(p9idg_plan* p, const float2* src, double* dst, size_t n) {
     return P9IDG_OK;
 }
 
+hipEvent_t pooled_event(p9idg_plan* p, size_t i) {
+    while (p->event_pool.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        p->event_pool.push_back(e);
+    }
+    return p->event_pool[i];
+}
+
+// Starts a timed span on the plan's stream when collection is on.
+int span_begin(p9idg_plan* p, int kind) {
+    if (!p->collect) return P9IDG_OK;
+    const size_t i = p->spans.size() * 2;
+    hipEvent_t a = pooled_event(p, i), b = pooled_event(p, i + 1);
+    if (!a || !b) return fail(P9IDG_ERR_HIP, "event creation failed");
+    P9_HIP(hipEventRecord(a, p->stream));
+    p->spans.push_back({a, b, kind});
+    return P9IDG_OK;
+}
+int span_end(p9idg_plan* p) {
+    if (!p->collect || p->spans.empty()) return P9IDG_OK;
+    P9_HIP(hipEventRecord(p->spans.back().stop, p->stream));
+    return P9IDG_OK;
+}
+
 KernelArgs make_args(const p9idg_plan* p, int mode) {
     KernelArgs a{};
     a.meta = p->d_meta;
@@ -220,12 +255,14 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     P9_HIP(hipEventRecord(p->ev[0], p->stream));
     P9_HIP(launch_fill_zero(target, n, p->stream));
     P9_HIP(hipEventRecord(p->ev[1], p->stream));
+    if (int st = span_begin(p, 0)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
         e = launch_grid_fp32(a, p->c.subgrid_size, p->stream);
     else
         e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
+    if (int st = span_end(p)) return st;
     P9_HIP(hipEventRecord(p->ev[2], p->stream));
     p->timed_grid = true;
     return P9IDG_OK;
@@ -239,12 +276,14 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     KernelArgs a = make_args(p, mode);
     P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));  // Gridder.cs:96
     P9_HIP(hipEventRecord(p->ev[3], p->stream));
+    if (int st = span_begin(p, 1)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
         e = launch_degrid_fp32(a, p->c.subgrid_size, p->stream);
     else
         e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, p->stream);
     if (e != hipSuccess) return hip_fail(e, "degridder launch");
+    if (int st = span_end(p)) return st;
     P9_HIP(hipEventRecord(p->ev[4], p->stream));
     p->timed_degrid = true;
     return P9IDG_OK;
@@ -447,6 +486,7 @@ int p9idg_destroy(p9idg_plan* p) {
     if (p->d_staging) (void)hipFree(p->d_staging);
     for (auto& ev : p->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : p->event_pool) (void)hipEventDestroy(ev);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return P9IDG_OK;
@@ -776,6 +816,15 @@ int p9idg_upload_model_grid(p9idg_plan* p, const void* grid) {
     return sync(p);
 }
 
+int p9idg_download_model_grid(p9idg_plan* p, void* out) {
+    if (int st = check_plan(p)) return st;
+    if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(out, p->d_model_grid, grid_elems(p) * sizeof(float2),
+                          hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
 int p9idg_download_model_vis(p9idg_plan* p, void* out) {
     if (int st = check_obs(p)) return st;
     if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
@@ -820,6 +869,35 @@ int p9idg_last_timings(p9idg_plan* p, float* grid_ms, float* degrid_ms, float* f
         *fft_ms = -1.f;
         if (p->timed_fft && hipEventElapsedTime(&ms, p->ev[5], p->ev[6]) == hipSuccess) *fft_ms = ms;
     }
+    return P9IDG_OK;
+}
+
+int p9idg_timing_reset(p9idg_plan* p, int32_t collect) {
+    if (int st = check_plan(p)) return st;
+    DeviceGuard dg(p->device);
+    P9_HIP(hipStreamSynchronize(p->stream));
+    p->spans.clear();
+    p->collect = collect != 0;
+    return P9IDG_OK;
+}
+
+int p9idg_timing_totals(p9idg_plan* p, double* grid_ms, int64_t* grid_launches,
+                        double* degrid_ms, int64_t* degrid_launches) {
+    if (int st = check_plan(p)) return st;
+    DeviceGuard dg(p->device);
+    P9_HIP(hipStreamSynchronize(p->stream));
+    double t[2] = {0.0, 0.0};
+    int64_t n[2] = {0, 0};
+    for (const auto& sp : p->spans) {
+        float ms = 0.f;
+        P9_HIP(hipEventElapsedTime(&ms, sp.start, sp.stop));
+        t[sp.kind] += ms;
+        n[sp.kind]++;
+    }
+    if (grid_ms) *grid_ms = t[0];
+    if (grid_launches) *grid_launches = n[0];
+    if (degrid_ms) *degrid_ms = t[1];
+    if (degrid_launches) *degrid_launches = n[1];
     return P9IDG_OK;
 }
 

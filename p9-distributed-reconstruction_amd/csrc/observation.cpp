@@ -1,4 +1,7 @@
 // Seeded synthetic observations (host).  Conventions are stated in p9obs.h.
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE
+#endif
 #include "p9obs.h"
 
 #include <cmath>
@@ -83,11 +86,24 @@ void p9obs_uvw(int32_t n_ant, const double* enu, double lat, double dec, int32_t
         }
 }
 
+// Phases of point sources are evaluated for all sources of one (baseline,
+// time) at once so the sincos calls vectorise; Gaussians (few) go one by one.
 void p9obs_sky_vis(int32_t B, int32_t T, int32_t C, const double* uvw, const double* freq,
                    int32_t n_src, const p9obs_source* src, double* vis, int32_t threads) {
-    std::vector<double> nm1(n_src);
-    for (int32_t s = 0; s < n_src; s++)
-        nm1[s] = std::sqrt(1.0 - src[s].l * src[s].l - src[s].m * src[s].m) - 1.0;
+    std::vector<double> pl, pm, pn, pf;  // point sources: l, m, n - 1, flux
+    std::vector<int32_t> gauss;
+    for (int32_t s = 0; s < n_src; s++) {
+        const double nm1 = std::sqrt(1.0 - src[s].l * src[s].l - src[s].m * src[s].m) - 1.0;
+        if (src[s].sigma > 0.0) {
+            gauss.push_back(s);
+        } else {
+            pl.push_back(src[s].l);
+            pm.push_back(src[s].m);
+            pn.push_back(nm1);
+            pf.push_back(src[s].flux);
+        }
+    }
+    const int32_t np = (int32_t)pl.size();
     // Evenly spaced channels (f_ch = f_0 + ch * df to 1e-12 relative) use the
     // rotation exp(-2 pi i d f_ch / c) = E_0 * R^ch with one sincos pair for E_0
     // and one for R; other frequency sets evaluate every channel directly.
@@ -98,44 +114,65 @@ void p9obs_sky_vis(int32_t B, int32_t T, int32_t C, const double* uvw, const dou
     const int64_t BT = (int64_t)B * T;
 #ifdef _OPENMP
     const int nt = threads > 0 ? threads : omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+#pragma omp parallel num_threads(nt)
 #endif
-    for (int64_t bt = 0; bt < BT; bt++) {
-        const double* p = uvw + bt * 3;
-        double* out = vis + bt * C * 2;
-        for (int32_t ch = 0; ch < C; ch++) {
-            out[2 * ch] = 0.0;
-            out[2 * ch + 1] = 0.0;
-        }
-        const double r2 = p[0] * p[0] + p[1] * p[1];
-        for (int32_t s = 0; s < n_src; s++) {
-            // geometric delay in metres; phase per channel = -2 pi d f / c
-            const double d = p[0] * src[s].l + p[1] * src[s].m + p[2] * nm1[s];
-            double er = 0, ei = 0, rr = 0, ri = 0;
+    {
+        std::vector<double> er(np), ei(np), rr(np), ri(np), ang(np);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int64_t bt = 0; bt < BT; bt++) {
+            const double* p = uvw + bt * 3;
+            double* out = vis + bt * C * 2;
+            const double u = p[0], v = p[1], w = p[2];
             if (even) {
-                const double a0 = -2.0 * kPi * d * (freq[0] / kC);
-                const double a1 = -2.0 * kPi * d * (df / kC);
-                er = std::cos(a0);
-                ei = std::sin(a0);
-                rr = std::cos(a1);
-                ri = std::sin(a1);
-            }
-            for (int32_t ch = 0; ch < C; ch++) {
-                const double k = freq[ch] / kC;
-                double amp = src[s].flux;
-                if (src[s].sigma > 0.0)
-                    amp *= std::exp(-2.0 * kPi * kPi * src[s].sigma * src[s].sigma * r2 * k * k);
-                if (!even) {
-                    const double ph = -2.0 * kPi * d * k;
-                    er = std::cos(ph);
-                    ei = std::sin(ph);
+                const double k0 = -2.0 * kPi * (freq[0] / kC), k1 = -2.0 * kPi * (df / kC);
+#pragma omp simd
+                for (int32_t s = 0; s < np; s++) {
+                    const double d = u * pl[s] + v * pm[s] + w * pn[s];
+                    er[s] = std::cos(d * k0);
+                    ei[s] = std::sin(d * k0);
+                    rr[s] = std::cos(d * k1);
+                    ri[s] = std::sin(d * k1);
                 }
-                out[2 * ch] += amp * er;
-                out[2 * ch + 1] += amp * ei;
-                if (even) {
-                    const double nr = er * rr - ei * ri;
-                    ei = er * ri + ei * rr;
-                    er = nr;
+                for (int32_t ch = 0; ch < C; ch++) {
+                    double sr = 0.0, si = 0.0;
+#pragma omp simd reduction(+ : sr, si)
+                    for (int32_t s = 0; s < np; s++) {
+                        sr += pf[s] * er[s];
+                        si += pf[s] * ei[s];
+                        const double nr = er[s] * rr[s] - ei[s] * ri[s];
+                        ei[s] = er[s] * ri[s] + ei[s] * rr[s];
+                        er[s] = nr;
+                    }
+                    out[2 * ch] = sr;
+                    out[2 * ch + 1] = si;
+                }
+            } else {
+                for (int32_t ch = 0; ch < C; ch++) {
+                    const double k = -2.0 * kPi * (freq[ch] / kC);
+                    double sr = 0.0, si = 0.0;
+#pragma omp simd reduction(+ : sr, si)
+                    for (int32_t s = 0; s < np; s++) {
+                        const double d = u * pl[s] + v * pm[s] + w * pn[s];
+                        sr += pf[s] * std::cos(d * k);
+                        si += pf[s] * std::sin(d * k);
+                    }
+                    out[2 * ch] = sr;
+                    out[2 * ch + 1] = si;
+                }
+            }
+            const double r2 = u * u + v * v;
+            for (int32_t s : gauss) {
+                const double nm1 = std::sqrt(1.0 - src[s].l * src[s].l - src[s].m * src[s].m) - 1.0;
+                const double d = u * src[s].l + v * src[s].m + w * nm1;
+                for (int32_t ch = 0; ch < C; ch++) {
+                    const double k = freq[ch] / kC;
+                    const double amp = src[s].flux * std::exp(-2.0 * kPi * kPi * src[s].sigma *
+                                                              src[s].sigma * r2 * k * k);
+                    const double ph = -2.0 * kPi * d * k;
+                    out[2 * ch] += amp * std::cos(ph);
+                    out[2 * ch + 1] += amp * std::sin(ph);
                 }
             }
         }

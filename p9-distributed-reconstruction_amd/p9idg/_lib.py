@@ -120,10 +120,13 @@ IDG_SIGNATURES = {
     "p9idg_device_buffers": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "p9idg_download_grid": [_vp, _i32, _vp],
     "p9idg_upload_model_grid": [_vp, _vp],
+    "p9idg_download_model_grid": [_vp, _vp],
     "p9idg_download_model_vis": [_vp, _vp],
     "p9idg_upload_image": [_vp, _vp],
     "p9idg_download_image": [_vp, _vp],
     "p9idg_last_timings": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "p9idg_timing_reset": [_vp, _i32],
+    "p9idg_timing_totals": [_vp, ctypes.POINTER(_f64), _pi64, ctypes.POINTER(_f64), _pi64],
     "p9idg_gridded_visibilities": [_vp, _pi64],
 }
 _RESTYPES = {"p9idg_last_error": ctypes.c_char_p}

@@ -288,6 +288,12 @@ class Plan:
         g = np.ascontiguousarray(grid, dtype=np.complex64)
         check(_lib.idg().p9idg_upload_model_grid(self._h, ptr(g)))
 
+    def download_model_grid(self):
+        G = self.c.GridSize
+        out = np.zeros((G, G), dtype=np.complex64)
+        check(_lib.idg().p9idg_download_model_grid(self._h, ptr(out)))
+        return out
+
     def download_model_vis(self):
         out = np.zeros(self.shape, dtype=np.complex64)
         check(_lib.idg().p9idg_download_model_vis(self._h, ptr(out)))
@@ -307,6 +313,16 @@ class Plan:
         g, d, f = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         check(_lib.idg().p9idg_last_timings(self._h, ctypes.byref(g), ctypes.byref(d), ctypes.byref(f)))
         return {"grid_ms": g.value, "degrid_ms": d.value, "fft_ms": f.value}
+
+    def timing_reset(self, collect=True):
+        check(_lib.idg().p9idg_timing_reset(self._h, 1 if collect else 0))
+
+    def timing_totals(self):
+        """(grid_ms_total, grid_launches, degrid_ms_total, degrid_launches)"""
+        g, d = ctypes.c_double(), ctypes.c_double()
+        ng, nd = ctypes.c_int64(), ctypes.c_int64()
+        check(_lib.idg().p9idg_timing_totals(self._h, ctypes.byref(g), ctypes.byref(ng), ctypes.byref(d), ctypes.byref(nd)))
+        return g.value, ng.value, d.value, nd.value
 
     @property
     def gridded_visibilities(self):

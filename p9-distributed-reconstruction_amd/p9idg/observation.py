@@ -23,6 +23,7 @@ Every workload has S-K margin 8 (kernel 8) and MaxTimesteps 1024
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -149,8 +150,15 @@ def small(instance=0, threads=0):
     return Observation("small", c, uvw, freq, vis, sources, seed, {"noise": 0.05, "vis_dtype": "complex128"})
 
 
-def large(instance=0, threads=0, n_points=500, n_blobs=20):
-    """3072^2, 64 antennas (45 core 1 km + 19 in 8 km), 2000 x 8 s, 8 ch 1.0-1.4 GHz, L=32."""
+def _cache_dir():
+    return os.environ.get("P9IDG_CACHE", "/tmp/p9idg_cache")
+
+
+def large(instance=0, threads=0, n_points=500, n_blobs=20, cache=True):
+    """3072^2, 64 antennas (45 core 1 km + 19 in 8 km), 2000 x 8 s, 8 ch 1.0-1.4 GHz, L=32.
+
+    The sky sum (32M visibilities x 520 sources) takes tens of seconds, so the
+    visibilities are cached under $P9IDG_CACHE (default /tmp/p9idg_cache)."""
     seed = 2000 + instance
     G, S, K, L = 3072, 32, 8, 32
     cell = 1.5 * ARCSEC
@@ -158,9 +166,28 @@ def large(instance=0, threads=0, n_points=500, n_blobs=20):
     uvw = _uvw(enu, 2000, 8.0)
     freq = _even_freq(1.0e9, 1.4e9, 8)
     sources = _point_sources(seed + 100, n_points, n_blobs, G, cell)
-    vis = _sky(uvw, freq, sources, threads)
-    _noise(vis, 0.01, seed + 500)
-    vis = vis.astype(np.complex64)
+    path = os.path.join(_cache_dir(), "large_vis_%d_%d_%d.npy" % (seed, n_points, n_blobs))
+    vis = None
+    if cache and os.path.exists(path):
+        try:
+            vis = np.load(path, allow_pickle=False)
+            if vis.shape != (uvw.shape[0], uvw.shape[1], len(freq)) or vis.dtype != np.complex64:
+                vis = None
+        except (OSError, ValueError):
+            vis = None
+    if vis is None:
+        vis = _sky(uvw, freq, sources, threads)
+        _noise(vis, 0.01, seed + 500)
+        vis = vis.astype(np.complex64)
+        if cache:
+            try:
+                os.makedirs(_cache_dir(), exist_ok=True)
+                tmp = path + ".%d.tmp" % os.getpid()
+                with open(tmp, "wb") as f:
+                    np.save(f, vis, allow_pickle=False)
+                os.replace(tmp, path)
+            except OSError:
+                pass
     wstep = _wstep(uvw, freq, L)
     c = GriddingConstants(vis.size, G, S, K, 1024, cell, L, wstep)
     return Observation("large", c, uvw, freq, vis, sources, seed, {"noise": 0.01, "vis_dtype": "complex64", "wstep": wstep})
