@@ -10,97 +10,10 @@
 //            -> Gridder.Backwards (Gridder.cs:91-160)
 // The grid's quadrant swap (FFT.Shift) is folded into the adder/splitter
 // addressing, so no separate shift pass touches the grid.
-#include "device_common.h"
+#include "device_addsplit.h"
 
 namespace p9idg {
 namespace dev {
-
-// Adds the (normalised) subgrid tile onto the grid.  `tile` is the subgrid
-// after the inverse FFT, indexed [y][x] in FFT order.
-template <int S>
-__device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
-                                      const float2* tile, float norm) {
-    const int G = a.G;
-    if (a.mode == P9IDG_GRID_PLAIN) {
-        // Adder.cs:22-47 (range validated on the host: 0 <= U,V < G-S)
-        for (int p = threadIdx.x; p < S * S; p += kThreads) {
-            const int y = p / S, x = p - (p / S) * S;
-            const int ys = (y + S / 2) % S, xs = (x + S / 2) % S;
-            float2 v = cmulf(tile[ys * S + xs], a.adder_phasor[p]);
-            v.x *= norm;
-            v.y *= norm;
-            float2* dst = a.grid + (size_t)shifted(g.V + y, G) * G + shifted(g.U + x, G);
-            unsafeAtomicAdd(&dst->x, v.x);
-            unsafeAtomicAdd(&dst->y, v.y);
-        }
-        return;
-    }
-    // AdderWStack.cs:29-74: negative w layers are mirrored and conjugated.
-    const bool neg = g.W < 0;
-    int X = g.U, Y = g.V, W = g.W;
-    if (neg) {
-        X = G - X - S + 1;
-        Y = G - Y - S + 1;
-        W = -W - 1;
-    }
-    if (!(X >= 1 && X < G - S && Y >= 1 && Y < G - S)) return;
-    if (W < 0 || W >= a.L) return;  // metadata validated on the host
-    float2* layer = a.grid + (a.mode == P9IDG_GRID_WSUM ? 0 : (size_t)W * G * G);
-    for (int p = threadIdx.x; p < S * S; p += kThreads) {
-        const int y = p / S, x = p - (p / S) * S;
-        const int y_ = neg ? S - 1 - y : y;
-        const int x_ = neg ? S - 1 - x : x;
-        const int ys = (y_ + S / 2) % S, xs = (x_ + S / 2) % S;
-        float2 v = cmulf(a.adder_phasor[y_ * S + x_], tile[ys * S + xs]);
-        v.x *= norm;
-        v.y *= norm;
-        if (neg) v.y = -v.y;
-        float2* dst = layer + (size_t)shifted(Y + y, G) * G + shifted(X + x, G);
-        unsafeAtomicAdd(&dst->x, v.x);
-        unsafeAtomicAdd(&dst->y, v.y);
-    }
-}
-
-// Reads the subgrid from the model grid into `tile` (FFT order), zeros where
-// the reference leaves the subgrid empty.
-template <int S>
-__device__ __forceinline__ void splitter(const KernelArgs& a, const SubgridGeom& g,
-                                         float2* tile) {
-    const int G = a.G;
-    if (a.mode == P9IDG_DEGRID_PLAIN) {
-        // Adder.cs:65-90
-        const bool ok = g.U >= 0 && g.U < G - S && g.V >= 0 && g.V < G - S;
-        for (int p = threadIdx.x; p < S * S; p += kThreads) {
-            const int y = p / S, x = p - (p / S) * S;
-            const int yd = (y + S / 2) % S, xd = (x + S / 2) % S;
-            float2 v = make_float2(0.f, 0.f);
-            if (ok) {
-                float2 gv = a.model_grid[(size_t)shifted(g.V + y, G) * G + shifted(g.U + x, G)];
-                v = cmulf(gv, conjf2(a.adder_phasor[p]));
-            }
-            tile[yd * S + xd] = v;
-        }
-        return;
-    }
-    // AdderWStack.cs:97-134: one grid; negative w reads the mirrored pixel
-    // conjugated; the range test uses the unmirrored corner.
-    const bool neg = g.W < 0;
-    const int X = g.U, Y = g.V;
-    const bool ok = X >= 1 && X < G - S && Y >= 1 && Y < G - S;
-    for (int p = threadIdx.x; p < S * S; p += kThreads) {
-        const int y = p / S, x = p - (p / S) * S;
-        const int yd = (y + S / 2) % S, xd = (x + S / 2) % S;
-        float2 v = make_float2(0.f, 0.f);
-        if (ok) {
-            const int xs = neg ? G - X - x : X + x;
-            const int ys = neg ? G - Y - y : Y + y;
-            float2 gv = a.model_grid[(size_t)shifted(ys, G) * G + shifted(xs, G)];
-            if (neg) gv.y = -gv.y;
-            v = cmulf(conjf2(a.adder_phasor[p]), gv);
-        }
-        tile[yd * S + xd] = v;
-    }
-}
 
 template <int S>
 __global__ __launch_bounds__(kThreads) void grid_fp32_kernel(KernelArgs a) {

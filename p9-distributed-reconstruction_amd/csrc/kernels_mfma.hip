@@ -1,7 +1,393 @@
-// Split-precision MFMA gridder / degridder (not built yet).
-#include "device_common.h"
+// Split-precision MFMA gridder and degridder (gfx950, 32x32x16 BF16/FP16
+// MFMA with FP32 accumulation), each fused with the LDS subgrid FFT and the
+// adder / splitter.  Same stages and semantics as kernels_fp32.hip
+// (Gridder.cs:15-160, SubgridFFT.cs, Adder.cs, AdderWStack.cs).
+//
+// The reference's phasor for visibility v and subgrid pixel (y, x) is
+//   exp(2 pi i (a_v l_x + b_v m_y + c_v n_xy))          (Gridder.cs:62-70)
+// with a, b, c the per-visibility coefficients in turns (vis_coeffs) and
+// n_xy = ComputeN(l_x, m_y).  ComputeN evaluates t / (t + sqrt(1 - t)),
+// t = l^2 + m^2 (Gridder.cs:172-173), which is t - t^2/2 + ...; splitting it
+// along the axes, n_xy = n(l_x, 0) + n(0, m_y) + r_xy with r_xy ~ -l^2 m^2
+// (below 1.6e-8 for a 3072 x 1.5" image), the phasor factors into
+//   A_v[x] = exp(2 pi i (a_v l_x + c_v n(l_x, 0)))   (one row of 32 per vis)
+//   B_v[y] = exp(2 pi i (b_v m_y + c_v n(0, m_y)))   (one column of 32 per vis)
+//   W_v[y,x] = exp(2 pi i c_v r_xy) = 1 + 2 pi i c_v r_xy + O((2 pi c_v r_xy)^2)
+// and one correction term keeps the error below (2 pi |c_v| r_max)^2 / 2.
+// The gridder becomes two complex GEMMs over the visibilities of a subgrid:
+//   M0[y][x] = sum_v (V_v B_v[y]) A_v[x],   M1[y][x] = sum_v (c_v V_v B_v[y]) A_v[x]
+//   subgrid[y][x] = M0 + 2 pi i r_xy M1
+// and the degridder, with P the split subgrid pixels,
+//   Q0[y][v] = sum_x P[y][x] conj(A_v[x]),  Q1[y][v] = sum_x r_xy P[y][x] conj(A_v[x])
+//   vis_v = scale * sum_y conj(B_v[y]) (Q0 - 2 pi i c_v Q1)
+// Phases are evaluated in FP64 (coefficients and per-entry phase), reduced to
+// [-0.5, 0.5] turns and passed to the hardware sin/cos.  The GEMM operands of
+// the main terms are split x = hi + lo in BF16 (or FP16) and multiplied as
+// hi*hi + hi*lo + lo*hi with FP32 accumulation; the correction terms use the
+// hi parts only.  FP16 operands are normalised by powers of two (exact) per
+// subgrid so the low parts stay in FP16's normal range.
+#include "device_addsplit.h"
 
 namespace p9idg {
-hipError_t launch_grid_mfma(const KernelArgs&, int, int, hipStream_t) { return hipErrorNotSupported; }
-hipError_t launch_degrid_mfma(const KernelArgs&, int, int, hipStream_t) { return hipErrorNotSupported; }
+namespace dev {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct PrecBF16 {
+    typedef __bf16 T;
+    typedef bf16x8 V;
+    static constexpr bool kNormalise = false;
+    __device__ static __forceinline__ f32x16 mma(V a, V b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+struct PrecF16 {
+    typedef _Float16 T;
+    typedef f16x8 V;
+    static constexpr bool kNormalise = true;
+    __device__ static __forceinline__ f32x16 mma(V a, V b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// x = hi + lo with hi = round(x), lo = round(x - hi) in the operand type.
+#define P9_SPLIT(P, x, VH, VL, e)                         \
+    do {                                                  \
+        const float x_ = (x);                             \
+        const typename P::T h_ = (typename P::T)x_;       \
+        (VH)[e] = h_;                                     \
+        (VL)[e] = (typename P::T)(x_ - (float)h_);        \
+    } while (0)
+
+// exp(2 pi i ph) for ph in turns (FP64), reduced to [-0.5, 0.5] before the
+// FP32 hardware sin/cos.
+__device__ __forceinline__ float2 cis_f64(double ph) {
+    const float r = (float)(ph - rint(ph));
+    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+}
+
+// Separable part of ComputeN along one axis: n(l, 0) = n(0, l).
+__device__ __forceinline__ double axis_n(int x, int S, float image_size) {
+    return (double)compute_n(compute_l(x, S, image_size), 0.f);
+}
+// r_xy = ComputeN(l_x, m_y) - n(l_x, 0) - n(0, m_y) (FP64 of the float n's).
+__device__ __forceinline__ double residual_n(int x, int y, int S, float image_size) {
+    const float l = compute_l(x, S, image_size), m = compute_l(y, S, image_size);
+    return (double)compute_n(l, m) - (double)compute_n(l, 0.f) - (double)compute_n(0.f, m);
+}
+
+// Power of two >= |x| (1 for x == 0); exact normaliser.
+__device__ __forceinline__ float pow2_ceil(float x) {
+    if (!(x > 0.f)) return 1.f;
+    return exp2f(ceilf(log2f(x)));
+}
+
+// Block-wide max over 256 threads (4 waves).
+__device__ __forceinline__ float block_max(float v, float* red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+constexpr double kInvTwoPi = 1.0 / kTwoPi;
+constexpr float kTwoPiF = 6.2831853071795864769f;
+
+// ---------------------------------------------------------------------------
+// Gridder.  One 256-thread workgroup per subgrid; the 4 waves split the
+// subgrid's visibilities (16 per MFMA step, 8 per lane group) and each keeps
+// a full 32 x 32 set of accumulators; the partial subgrids are summed in LDS.
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
+    static_assert(S <= 32, "one 32x32 MFMA tile per subgrid");
+    typedef typename P::T T;
+    typedef typename P::V V;
+    constexpr int ROUND = kThreads;
+    __shared__ double s_a[ROUND], s_b[ROUND], s_c[ROUND];
+    __shared__ float2 s_v[ROUND];
+    __shared__ float2 s_tile[2][S * S];
+    __shared__ float s_red[4];
+
+    const int sidx = a.order[blockIdx.x];
+    const SubgridGeom g = load_geom(a, sidx, S);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 31, grp = lane >> 5;
+    const int C = a.C, K = g.nt * C;
+    const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
+
+    for (int p = tid; p < S * S; p += kThreads) s_tile[0][p] = make_float2(0.f, 0.f);
+    __syncthreads();
+
+    // FP16: scale the visibilities so |V| <= 1, and c * L by 2^-16.
+    float vscale = 1.f;
+    float cscale = 1.f;
+    if (P::kNormalise) {
+        float m = 0.f;
+        for (int j = tid; j < K; j += kThreads) {
+            const float2 v = a.vis[vbase + j];
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+        vscale = 1.f / pow2_ceil(block_max(m, s_red));
+        cscale = 1.f / 65536.f;
+    }
+
+    // lane's row y and column x are both li
+    const double lx = (double)compute_l(li, S, a.image_size);
+    const double nx = axis_n(li, S, a.image_size);
+
+    f32x16 acc0r = {}, acc0i = {}, acc1r = {}, acc1i = {};
+
+    for (int j0 = 0; j0 < K; j0 += ROUND) {
+        {
+            const int j = j0 + tid;
+            if (j < K) {
+                const int t = g.t0 + j / C, ch = j % C;
+                const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
+                const double k = a.wavenumbers[ch];
+                s_a[tid] = (g.u_off - k * p[0]) * kInvTwoPi;
+                s_b[tid] = (g.v_off - k * p[1]) * kInvTwoPi;
+                s_c[tid] = (g.w_off - k * p[2]) * kInvTwoPi;
+                const float2 v = a.vis[vbase + j];
+                s_v[tid] = make_float2(v.x * vscale, v.y * vscale);
+            } else {
+                s_a[tid] = 0.0;
+                s_b[tid] = 0.0;
+                s_c[tid] = 0.0;
+                s_v[tid] = make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+        const int nsteps = (min(ROUND, K - j0) + 15) / 16;
+        for (int st = wave; st < nsteps; st += 4) {
+            V Lrh, Lrl, Lih, Lil, cLr, cLi, Arh, Arl, Aih, Ail;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int j = st * 16 + grp * 8 + e;
+                const double ca = s_a[j], cb = s_b[j], cc = s_c[j];
+                const float2 v = s_v[j];
+                const double cn = cc * nx;
+                const float2 A = cis_f64(fma(ca, lx, cn));
+                const float2 B = cis_f64(fma(cb, lx, cn));
+                const float2 L = cmulf(v, B);
+                const float cf = (float)cc * cscale;
+                P9_SPLIT(P, L.x, Lrh, Lrl, e);
+                P9_SPLIT(P, L.y, Lih, Lil, e);
+                P9_SPLIT(P, A.x, Arh, Arl, e);
+                P9_SPLIT(P, A.y, Aih, Ail, e);
+                cLr[e] = (T)(cf * L.x);
+                cLi[e] = (T)(cf * L.y);
+            }
+            const V nLih = -Lih, nLil = -Lil, ncLi = -cLi;
+            // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi)
+            acc0r = P::mma(Lrh, Arh, acc0r);
+            acc0i = P::mma(Lrh, Aih, acc0i);
+            acc1r = P::mma(cLr, Arh, acc1r);
+            acc1i = P::mma(cLr, Aih, acc1i);
+            acc0r = P::mma(Lrh, Arl, acc0r);
+            acc0i = P::mma(Lrh, Ail, acc0i);
+            acc1r = P::mma(ncLi, Aih, acc1r);
+            acc1i = P::mma(cLi, Arh, acc1i);
+            acc0r = P::mma(Lrl, Arh, acc0r);
+            acc0i = P::mma(Lrl, Aih, acc0i);
+            acc0r = P::mma(nLih, Aih, acc0r);
+            acc0i = P::mma(Lih, Arh, acc0i);
+            acc0r = P::mma(nLih, Ail, acc0r);
+            acc0i = P::mma(Lih, Arl, acc0i);
+            acc0r = P::mma(nLil, Aih, acc0r);
+            acc0i = P::mma(Lil, Arh, acc0i);
+        }
+        __syncthreads();
+    }
+
+    // subgrid = (M0 + 2 pi i r M1) / vscale, summed over the 4 waves into the
+    // ifftshift-ed tile (Gridder.cs:76-79; identity taper)
+    const float back = 1.f / vscale;
+    const float back1 = back / cscale;
+    for (int w = 0; w < 4; w++) {
+        if (wave == w) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int y = 8 * (r / 4) + 4 * grp + (r % 4), x = li;
+                if (y < S && x < S) {
+                    const float rr = (float)(kTwoPi * residual_n(x, y, S, a.image_size));
+                    const float m1r = acc1r[r] * back1, m1i = acc1i[r] * back1;
+                    const float gr = acc0r[r] * back - rr * m1i;
+                    const float gi = acc0i[r] * back + rr * m1r;
+                    float2& d = s_tile[0][((y + S / 2) % S) * S + (x + S / 2) % S];
+                    d.x += gr;
+                    d.y += gi;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, +1);
+    adder<S>(a, g, res, 1.0f / (float)(S * S));
+}
+
+// ---------------------------------------------------------------------------
+// Degridder.  One workgroup per subgrid: splitter + forward FFT into LDS, the
+// subgrid pixels split into registers as the MFMA A operand (rows y, k = x),
+// then each wave takes tiles of 32 visibilities (the N dimension).
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
+    static_assert(S <= 32, "one 32x32 MFMA tile per subgrid");
+    typedef typename P::V V;
+    __shared__ float2 s_tile[2][S * S];
+    __shared__ double s_l[32], s_n1[32];
+    __shared__ float s_red[4];
+
+    const int sidx = a.order[blockIdx.x];
+    const SubgridGeom g = load_geom(a, sidx, S);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 31, grp = lane >> 5;
+    const int C = a.C, K = g.nt * C;
+    const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
+
+    splitter<S>(a, g, s_tile[0]);
+    if (tid < 32) {
+        s_l[tid] = (double)compute_l(tid, S, a.image_size);
+        s_n1[tid] = axis_n(tid, S, a.image_size);
+    }
+    __syncthreads();
+    const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, -1);
+
+    // normalisers (powers of two): pixels to |P| <= 1 (FP16 only), r to <= 1
+    float pscale = 1.f;
+    if (P::kNormalise) {
+        float m = 0.f;
+        for (int p = tid; p < S * S; p += kThreads) m = fmaxf(m, fmaxf(fabsf(res[p].x), fabsf(res[p].y)));
+        pscale = 1.f / pow2_ceil(block_max(m, s_red));
+    }
+    const double r_corner = residual_n(0, 0, S, a.image_size);
+    const float rscale = 1.f / pow2_ceil((float)fabs(r_corner));
+
+    // A operand: pixels_copy[y][x] = subgrid[(y+S/2)%S][(x+S/2)%S] (Gridder.cs:101-117)
+    // lane holds row y = li, k = x = 16 h + 8 grp + e
+    V Prh[2], Prl[2], Pih[2], Pil[2], nPrh[2], nPrl[2], Qr[2], Qi[2], nQr[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int y = li, x = 16 * h + 8 * grp + e;
+            float2 pv = make_float2(0.f, 0.f);
+            float rr = 0.f;
+            if (y < S && x < S) {
+                pv = res[((y + S / 2) % S) * S + (x + S / 2) % S];
+                pv.x *= pscale;
+                pv.y *= pscale;
+                rr = (float)residual_n(x, y, S, a.image_size) * rscale;
+            }
+            P9_SPLIT(P, pv.x, Prh[h], Prl[h], e);
+            P9_SPLIT(P, pv.y, Pih[h], Pil[h], e);
+            Qr[h][e] = (typename P::T)(rr * pv.x);
+            Qi[h][e] = (typename P::T)(rr * pv.y);
+        }
+        nPrh[h] = -Prh[h];
+        nPrl[h] = -Prl[h];
+        nQr[h] = -Qr[h];
+    }
+    // this lane's rows of the output tile and their (m, m^2/2)
+    const float out_scale = a.degrid_scale / pscale;
+    const float q1_back = kTwoPiF / rscale;
+
+    const int ntiles = (K + 31) / 32;
+    for (int vt = wave; vt < ntiles; vt += 4) {
+        const int j = vt * 32 + li;
+        const int jc = j < K ? j : K - 1;
+        const int t = g.t0 + jc / C, ch = jc % C;
+        const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
+        const double k = a.wavenumbers[ch];
+        const double ca = (g.u_off - k * p[0]) * kInvTwoPi;
+        const double cb = (g.v_off - k * p[1]) * kInvTwoPi;
+        const double cc = (g.w_off - k * p[2]) * kInvTwoPi;
+
+        // B operand: A_v[x] for this lane's vis (column) and x = 16 h + 8 grp + e
+        V Arh[2], Arl[2], Aih[2], Ail[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int x = 16 * h + 8 * grp + e;
+                const float2 A = cis_f64(fma(ca, s_l[x], cc * s_n1[x]));
+                P9_SPLIT(P, A.x, Arh[h], Arl[h], e);
+                P9_SPLIT(P, A.y, Aih[h], Ail[h], e);
+            }
+        }
+        f32x16 q0r = {}, q0i = {}, q1r = {}, q1i = {};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar - Pr Ai
+            q0r = P::mma(Prh[h], Arh[h], q0r);
+            q0i = P::mma(Pih[h], Arh[h], q0i);
+            q1r = P::mma(Qr[h], Arh[h], q1r);
+            q1i = P::mma(Qi[h], Arh[h], q1i);
+            q0r = P::mma(Prh[h], Arl[h], q0r);
+            q0i = P::mma(Pih[h], Arl[h], q0i);
+            q1r = P::mma(Qi[h], Aih[h], q1r);
+            q1i = P::mma(nQr[h], Aih[h], q1i);
+            q0r = P::mma(Prl[h], Arh[h], q0r);
+            q0i = P::mma(Pil[h], Arh[h], q0i);
+            q0r = P::mma(Pih[h], Aih[h], q0r);
+            q0i = P::mma(nPrh[h], Aih[h], q0i);
+            q0r = P::mma(Pih[h], Ail[h], q0r);
+            q0i = P::mma(nPrh[h], Ail[h], q0i);
+            q0r = P::mma(Pil[h], Aih[h], q0r);
+            q0i = P::mma(nPrl[h], Aih[h], q0i);
+        }
+        // vis = sum_y conj(B_v[y]) (Q0 - 2 pi i c Q1) over this lane's 16 rows
+        const float cf = (float)cc * q1_back;
+        float sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = 8 * (r / 4) + 4 * grp + (r % 4);
+            if (y < S) {
+                const float2 B = cis_f64(fma(cb, s_l[y], cc * s_n1[y]));
+                const float zr = q0r[r] + cf * q1i[r];
+                const float zi = q0i[r] - cf * q1r[r];
+                sr = fmaf(B.x, zr, fmaf(B.y, zi, sr));
+                si = fmaf(B.x, zi, fmaf(-B.y, zr, si));
+            }
+        }
+        sr += __shfl_xor(sr, 32);
+        si += __shfl_xor(si, 32);
+        if (grp == 0 && j < K) a.vis_out[vbase + j] = make_float2(sr * out_scale, si * out_scale);
+    }
+}
+
+}  // namespace dev
+
+#define P9_LAUNCH_MFMA(KERNEL, S, PREC) \
+    hipLaunchKernelGGL((KERNEL<S, PREC>), dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a)
+
+hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st) {
+    if (a.nr_subgrids <= 0) return hipSuccess;
+    const bool f16 = precision == P9IDG_KERNEL_MFMA_FP16;
+    switch (S) {
+        case 8: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 8, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 8, dev::PrecBF16); break;
+        case 16: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 16, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 16, dev::PrecBF16); break;
+        case 24: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 24, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 24, dev::PrecBF16); break;
+        case 32: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 32, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 32, dev::PrecBF16); break;
+        default: return hipErrorNotSupported;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st) {
+    if (a.nr_subgrids <= 0) return hipSuccess;
+    const bool f16 = precision == P9IDG_KERNEL_MFMA_FP16;
+    switch (S) {
+        case 8: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 8, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 8, dev::PrecBF16); break;
+        case 16: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 16, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 16, dev::PrecBF16); break;
+        case 24: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 24, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 24, dev::PrecBF16); break;
+        case 32: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 32, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 32, dev::PrecBF16); break;
+        default: return hipErrorNotSupported;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace p9idg

@@ -1,0 +1,259 @@
+"""GPU parity: every gridding entry point of libp9idg (through the C ABI)
+against the FP64 oracle on the same seeded inputs, for the FP32 vector kernel
+and the split BF16 / FP16 MFMA kernels.
+
+Tolerance: max |gpu - oracle| / max |oracle| <= 1e-5 for grids, images and
+degridded visibilities (the accuracy bar of the split-precision form).  The
+"large" workload is checked on a stride sample of its subgrids, and at full
+size through the adjoint identity <Grid(V), X> = <V, DeGrid(X)>.
+"""
+import numpy as np
+import pytest
+
+import p9idg
+from p9idg import observation as obs
+
+from conftest import oracle_constants, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["fp32", "mfma_bf16", "mfma_fp16"]
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def small():
+    ob = obs.small()
+    return ob, ob.partition(), oracle_constants(ob.c)
+
+
+@pytest.fixture(scope="module")
+def small_ref(small, O):
+    ob, part, oc = small
+    m = part.subgrids
+    ref = {}
+    ref["grid"] = O.grid(oc, m, ob.vis, ob.uvw, ob.freq)
+    ref["gridw"] = O.gridw(oc, m, ob.vis, ob.uvw, ob.freq)
+    ref["degrid"] = O.degrid(oc, m, ref["grid"], ob.uvw, ob.freq)
+    ref["degridw"] = O.degridw(oc, m, ref["grid"], ob.uvw, ob.freq)
+    ref["to_image"] = O.to_image(oc, m, ob.vis, ob.uvw, ob.freq)
+    ref["gridw_image"] = O.gridw_to_image(oc, m, ob.vis, ob.uvw, ob.freq)
+    ref["image_to_vis"] = O.image_to_vis(oc, m, ref["gridw_image"], ob.uvw, ob.freq)
+    ones = np.ones_like(ob.vis)
+    ref["psf_grid"] = O.grid(oc, m, ones, ob.uvw, ob.freq)
+    ref["psf"] = O.to_image(oc, m, ones, ob.uvw, ob.freq)
+    return ref
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_small_workload_every_entry_point(small, small_ref, kernel):
+    ob, part, oc = small
+    ref = small_ref
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        errs = {
+            "grid": rel_err(plan.grid(ob.vis), ref["grid"]),
+            "gridw": rel_err(plan.gridw(ob.vis), ref["gridw"]),
+            "degrid": rel_err(plan.degrid(ref["grid"]), ref["degrid"]),
+            "degridw": rel_err(plan.degridw(ref["grid"]), ref["degridw"]),
+            "to_image": rel_err(plan.to_image(ob.vis), ref["to_image"]),
+            "gridw_image": rel_err(plan.gridw_to_image(ob.vis), ref["gridw_image"]),
+            "image_to_vis": rel_err(plan.image_to_vis(ref["gridw_image"]), ref["image_to_vis"]),
+            "psf_grid": rel_err(plan.grid_psf(None), ref["psf_grid"]),
+            "psf": rel_err(plan.calculate_psf(None), ref["psf"]),
+        }
+    print(kernel, {k: "%.2e" % v for k, v in errs.items()})
+    assert max(errs.values()) <= TOL, errs
+
+
+def test_static_drop_ins_match_oracle(small, small_ref):
+    ob, part, oc = small
+    c = ob.c
+    assert rel_err(p9idg.IDG.Grid(c, part, ob.vis, ob.uvw, ob.freq), small_ref["grid"]) <= TOL
+    layers = p9idg.IDG.GridW(c, part, ob.vis, ob.uvw, ob.freq)
+    assert len(layers) == c.WLayerCount
+    assert rel_err(np.stack(layers), small_ref["gridw"]) <= TOL
+    grid = small_ref["grid"].copy()
+    vis = p9idg.IDG.DeGridW(c, part, grid, ob.uvw, ob.freq)
+    assert np.array_equal(grid, small_ref["grid"])  # the caller's grid is not modified
+    assert rel_err(vis, small_ref["degridw"]) <= TOL
+    assert rel_err(p9idg.IDG.ToImage(c, part, ob.vis, ob.uvw, ob.freq), small_ref["to_image"]) <= TOL
+    assert rel_err(p9idg.IDG.CalculatePSF(c, part, ob.uvw, np.zeros(ob.vis.shape, bool), ob.freq), small_ref["psf"]) <= TOL
+
+
+def test_psf_flags(small, O):
+    ob, part, oc = small
+    rng = np.random.default_rng(9)
+    flags = rng.random(ob.vis.shape) < 0.2
+    psf_vis = np.where(flags, 0, 1).astype(np.complex128)
+    ref = O.grid(oc, part.subgrids, psf_vis, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        assert rel_err(plan.grid_psf(flags), ref) <= TOL
+
+
+CASES = [
+    # S, K, L, wstep-from-data, antennas, timesteps, channels
+    (8, 4, 1, False, 8, 30, 5),
+    (16, 4, 4, True, 10, 40, 6),
+    (24, 8, 4, True, 10, 40, 6),
+    (32, 8, 4, True, 12, 50, 8),
+    (48, 8, 2, True, 8, 30, 4),
+    (64, 8, 2, True, 8, 30, 4),
+]
+
+
+@pytest.mark.parametrize("S,K,L,wst,n_ant,T,C", CASES)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_subgrid_sizes_and_wstack(O, S, K, L, wst, n_ant, T, C, kernel):
+    if kernel != "fp32" and S > 32:
+        pytest.skip("MFMA kernels cover subgrids up to 32")
+    ob = obs.custom(G=256, S=S, K=K, L=L, n_ant=n_ant, T=T, C=C, seed=40 + S, core_diam=1500.0,
+                    sources=[(1.0, 12 * obs.ARCSEC * 1.5, -9 * obs.ARCSEC * 1.5, 0.0), (0.3, -40 * obs.ARCSEC * 1.5, 5 * obs.ARCSEC * 1.5, 0.0)],
+                    noise=0.01, wstep=None if wst else 0.0)
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    m = part.subgrids
+    if L > 1:
+        assert (m["WLambda"] < 0).any() or (m["WLambda"] > 0).any()
+    ref_l = O.gridw(oc, m, ob.vis, ob.uvw, ob.freq)
+    model = ref_l.sum(axis=0)
+    ref_v = O.degridw(oc, m, model, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        e_g = rel_err(plan.gridw(ob.vis), ref_l)
+        e_d = rel_err(plan.degridw(model), ref_v)
+    print(kernel, S, "gridw %.2e degridw %.2e" % (e_g, e_d))
+    assert e_g <= TOL and e_d <= TOL
+
+
+def test_invalid_grid_error_and_wstack_skip(O):
+    ob = obs.custom(G=128, S=16, K=4, L=1, n_ant=6, T=10, C=3, seed=5)
+    part = ob.partition()
+    meta = part.subgrids.copy()
+    meta[0]["UPixel"] = ob.c.GridSize - ob.c.SubgridSize  # Adder.cs:30-31 rejects, AdderWStack skips
+    oc = oracle_constants(ob.c)
+    with p9idg.Plan(ob.c) as plan:
+        plan.set_observation(ob.uvw, ob.freq, meta)
+        with pytest.raises(p9idg.P9IDGError, match="invalid grid") as e:
+            plan.grid(ob.vis)
+        assert e.value.status == p9idg._lib.ERR_INVALID_GRID
+        assert rel_err(plan.gridw(ob.vis), O.gridw(oc, meta, ob.vis, ob.uvw, ob.freq)) <= TOL
+
+
+def test_bad_metadata_rejected():
+    ob = obs.custom(G=128, S=16, K=4, L=2, n_ant=6, T=10, C=3, seed=6)
+    part = ob.partition()
+    with p9idg.Plan(ob.c) as plan:
+        for field, value in [("baselineIdx", 999), ("timeSampleStart", 9), ("WLambda", 5)]:
+            meta = part.subgrids.copy()
+            meta[0][field] = value
+            with pytest.raises(p9idg.P9IDGError) as e:
+                plan.set_observation(ob.uvw, ob.freq, meta)
+            assert e.value.status == p9idg._lib.ERR_ARGUMENT
+
+
+def test_empty_partition_and_single_timestep_subgrids(O):
+    ob = obs.custom(G=128, S=16, K=4, L=1, n_ant=6, T=10, C=3, seed=7)
+    oc = oracle_constants(ob.c)
+    empty = np.zeros(0, dtype=p9idg.SUBGRID_DTYPE)
+    single = ob.partition().subgrids.copy()
+    single["timeSampleCount"] = 1
+    with p9idg.Plan(ob.c, kernel="mfma_bf16") as plan:
+        plan.set_observation(ob.uvw, ob.freq, empty)
+        assert np.abs(plan.gridw(ob.vis)).max() == 0
+        assert np.abs(plan.degridw(np.ones((128, 128)))).max() == 0
+        plan.set_observation(ob.uvw, ob.freq, single)
+        assert rel_err(plan.gridw(ob.vis), O.gridw(oc, single, ob.vis, ob.uvw, ob.freq)) <= TOL
+
+
+def test_fft_steps(O):
+    rng = np.random.default_rng(3)
+    G, L, N = 256, 3, 4321
+    layers = rng.standard_normal((L, G, G)) + 1j * rng.standard_normal((L, G, G))
+    got = p9idg.FFT.WStackIFFTFloat([layers[k] for k in range(L)], N)
+    ref = O.wstack_ifft_float(layers, N)
+    assert rel_err(got, ref) <= TOL
+    img = rng.standard_normal((G, G)).astype(np.float32)
+    assert rel_err(p9idg.FFT.Forward(img, 3.0), O.fft_forward_float(img, 3.0)) <= TOL
+    grid = layers[0]
+    assert rel_err(p9idg.FFT.Backward(grid, N), O.fft_backward(grid, N)) <= TOL
+
+
+def test_c64_and_c128_inputs_agree(small):
+    ob, part, oc = small
+    with p9idg.Plan(ob.c) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        a = plan.gridw(ob.vis.astype(np.complex64))
+        b = plan.gridw(ob.vis)
+    assert rel_err(a, b) <= 1e-6
+
+
+@pytest.fixture(scope="module")
+def large():
+    return obs.large()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_large_workload_sample(O, large, kernel):
+    ob = large
+    part = ob.partition()
+    meta = np.ascontiguousarray(part.subgrids[::97])
+    oc = oracle_constants(ob.c)
+    vis128 = ob.vis.astype(np.complex128)
+    ref_l = O.gridw(oc, meta, vis128, ob.uvw, ob.freq)
+    model = ref_l.sum(axis=0)
+    ref_v = O.degridw(oc, meta, model, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, meta)
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        layers = plan.download_grid(p9idg.GRID_WSTACK)
+        plan.upload_model_grid(model.astype(np.complex64))
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        vis = plan.download_model_vis()
+    e_g = rel_err(layers, ref_l)
+    covered = np.zeros(vis.shape[:2], bool)
+    for s in meta:
+        covered[s["baselineIdx"], s["timeSampleStart"]:s["timeSampleStart"] + s["timeSampleCount"]] = True
+    e_d = float(np.abs(vis - ref_v)[covered].max() / np.abs(ref_v[covered]).max())
+    print(kernel, "large sample (%d subgrids): gridw %.2e degridw %.2e" % (len(meta), e_g, e_d))
+    assert e_g <= TOL and e_d <= TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_large_workload_adjoint(large, kernel):
+    """<Grid(V), X> = <V, DeGrid(X)> at full size (S = 32: 1/S^2 exact)."""
+    ob = large
+    part = ob.partition()
+    rng = np.random.default_rng(1)
+    G = ob.c.GridSize
+    X = (rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))).astype(np.complex64)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_PLAIN)
+        grid = plan.download_grid(p9idg.GRID_PLAIN)
+        plan.upload_model_grid(X)
+        plan.exec_degrid(p9idg.DEGRID_PLAIN)
+        dv = plan.download_model_vis()
+    lhs = np.vdot(grid.astype(np.complex128), X.astype(np.complex128))
+    rhs = np.vdot(ob.vis.astype(np.complex128), dv.astype(np.complex128))
+    print(kernel, "adjoint rel diff %.2e" % (abs(lhs - rhs) / abs(lhs)))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-4
+
+
+def test_wsum_mode_matches_wstack_image(large):
+    """GRID_WSUM (all layers in one grid) gives the image of GRID_WSTACK."""
+    ob = large
+    part = ob.partition()
+    with p9idg.Plan(ob.c, kernel="mfma_bf16") as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        a = plan.download_image()
+        plan.exec_grid(p9idg.GRID_WSUM)
+        plan.exec_grid_to_image(p9idg.GRID_WSUM)
+        b = plan.download_image()
+    assert rel_err(b, a) <= 1e-5
