@@ -15,6 +15,9 @@
 //   * System.Numerics.Complex multiply: (ac - bd) + (bc + ad)i
 // FFTW is replaced by an exact-twiddle mixed-radix FP64 FFT with FFTW's sign
 // and normalisation conventions (FORWARD = e^-, BACKWARD = e^+, unnormalised).
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE
+#endif
 #include "oracle.h"
 
 #include <algorithm>
@@ -73,6 +76,14 @@ inline int32_t cs_dtoi(double x) {
 inline double cs_round(double x) {
     std::fesetround(FE_TONEAREST);
     return std::nearbyint(x);
+}
+
+// (cos x, sin x) of System.Math.Cos/Sin.  glibc's sincos returns the same
+// values as separate cos and sin calls (orc_sincos_mismatches checks it).
+inline cpx cis(double x) {
+    cpx r;
+    sincos(x, &r.im, &r.re);
+    return r;
 }
 
 int nthreads() {
@@ -215,6 +226,21 @@ static void shift_impl(int32_t g, T* a) {
 extern "C" {
 
 void orc_set_threads(int n) { g_threads = n; }
+
+int64_t orc_sincos_mismatches(int64_t n, uint64_t seed) {
+    int64_t bad = 0;
+    uint64_t z = seed;
+    for (int64_t i = 0; i < n; i++) {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t r = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+        r ^= r >> 31;
+        const double x = ((double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5) * 2.0e5;
+        const cpx c = cis(x);
+        if (c.re != std::cos(x) || c.im != std::sin(x)) bad++;
+    }
+    return bad;
+}
 int orc_get_threads(void) { return nthreads(); }
 
 // ---------------------------------------------------------------------------
@@ -364,7 +390,7 @@ void orc_gridder_forward(const orc_constants* c, const orc_subgrid* meta, int64_
                     double phaseOffset = uOffset * l + vOffset * mm + wOffset * nn;
                     for (int ch = 0; ch < C; ch++) {
                         double phase = phaseOffset - (phaseIndex * k[ch]);
-                        cpx phasor{std::cos(phase), std::sin(phase)};
+                        cpx phasor = cis(phase);
                         cpx vv = V[((size_t)m.baseline_idx * T + t) * C + ch];
                         pixel = cadd(pixel, cmul(vv, phasor));
                     }
@@ -417,7 +443,7 @@ void orc_gridder_backward(const orc_constants* c, const orc_subgrid* meta, int64
                         double phaseIndex = u * l + v * mm + w * nn;
                         double phaseOffset = uOffset * l + vOffset * mm + wOffset * nn;
                         double phase = (phaseIndex * k[ch]) - phaseOffset;
-                        cpx phasor{std::cos(phase), std::sin(phase)};
+                        cpx phasor = cis(phase);
                         visibility = cadd(visibility, cmul(px[(size_t)y * S + x], phasor));
                     }
                 out[((size_t)m.baseline_idx * T + t) * C + ch] = cscale(visibility, scale);

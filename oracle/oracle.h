@@ -54,6 +54,8 @@ enum {
 };
 
 void orc_set_threads(int n);
+/* number of x in a pseudo-random sample where sincos(x) differs from (sin x, cos x) */
+int64_t orc_sincos_mismatches(int64_t n, uint64_t seed);
 int orc_get_threads(void);
 
 /* Partitioner.cs:20-124.  Subgrids in baseline order, then time order.

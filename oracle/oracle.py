@@ -83,6 +83,13 @@ def get_threads():
     return lib().orc_get_threads()
 
 
+def sincos_mismatches(n, seed=1):
+    f = lib().orc_sincos_mismatches
+    f.restype = ctypes.c_int64
+    f.argtypes = [ctypes.c_int64, ctypes.c_uint64]
+    return f(n, seed)
+
+
 def _cc(c):
     return ctypes.byref(c)
 

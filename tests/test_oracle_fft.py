@@ -78,3 +78,8 @@ def test_substract(O):
     out = O.substract(v0, v1, flags)
     assert np.array_equal(out[flags], np.zeros(flags.sum(), dtype=np.complex128))
     assert np.array_equal(out[~flags], (v0 - v1)[~flags])
+
+
+def test_oracle_sincos_equals_sin_and_cos(O):
+    # the oracle evaluates Math.Cos/Math.Sin pairs with sincos
+    assert O.sincos_mismatches(2_000_000, 7) == 0
