@@ -497,8 +497,6 @@ int p9idg_set_kernel(p9idg_plan* p, int32_t kernel) {
     if (kernel != P9IDG_KERNEL_FP32 && kernel != P9IDG_KERNEL_MFMA_BF16 &&
         kernel != P9IDG_KERNEL_MFMA_FP16)
         return fail(P9IDG_ERR_ARGUMENT, "unknown kernel");
-    if (kernel != P9IDG_KERNEL_FP32 && p->c.subgrid_size > 32)
-        return fail(P9IDG_ERR_UNSUPPORTED, "the MFMA kernels support subgrid sizes up to 32");
     p->kernel = kernel;
     return P9IDG_OK;
 }

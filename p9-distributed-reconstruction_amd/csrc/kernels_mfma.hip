@@ -98,12 +98,16 @@ constexpr double kInvTwoPi = 1.0 / kTwoPi;
 constexpr float kTwoPiF = 6.2831853071795864769f;
 
 // ---------------------------------------------------------------------------
-// Gridder.  One 256-thread workgroup per subgrid; the 4 waves split the
-// subgrid's visibilities (16 per MFMA step, 8 per lane group) and each keeps
-// a full 32 x 32 set of accumulators; the partial subgrids are summed in LDS.
+// Gridder.  One 256-thread workgroup per subgrid.  For S <= 32 the subgrid is
+// one 32 x 32 MFMA tile and the 4 waves split the subgrid's visibilities (16
+// per MFMA step, 8 per lane group), each keeping a full set of accumulators;
+// for S = 48, 64 the subgrid is 2 x 2 tiles and wave w owns tile w over all
+// visibilities.  Partial subgrids are summed in LDS.
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
-    static_assert(S <= 32, "one 32x32 MFMA tile per subgrid");
+    constexpr int NT = (S + 31) / 32;  // tiles per axis
+    constexpr int WPK = 4 / (NT * NT); // waves sharing one tile's visibilities
+    static_assert(NT * 32 >= S && NT <= 2, "subgrids up to 64");
     typedef typename P::T T;
     typedef typename P::V V;
     constexpr int ROUND = kThreads;
@@ -135,9 +139,13 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
         cscale = 1.f / 65536.f;
     }
 
-    // lane's row y and column x are both li
-    const double lx = (double)compute_l(li, S, a.image_size);
-    const double nx = axis_n(li, S, a.image_size);
+    // this wave's tile and the lane's row y (A operand) and column x (B operand)
+    const int tile = wave / WPK, kpart = wave % WPK;
+    const int ty = tile / NT, tx = tile % NT;
+    const double lrow = (double)compute_l(ty * 32 + li, S, a.image_size);
+    const double nrow = axis_n(ty * 32 + li, S, a.image_size);
+    const double lcol = (double)compute_l(tx * 32 + li, S, a.image_size);
+    const double ncol = axis_n(tx * 32 + li, S, a.image_size);
 
     f32x16 acc0r = {}, acc0i = {}, acc1r = {}, acc1i = {};
 
@@ -162,16 +170,15 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
         }
         __syncthreads();
         const int nsteps = (min(ROUND, K - j0) + 15) / 16;
-        for (int st = wave; st < nsteps; st += 4) {
+        for (int st = kpart; st < nsteps; st += WPK) {
             V Lrh, Lrl, Lih, Lil, cLr, cLi, Arh, Arl, Aih, Ail;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int j = st * 16 + grp * 8 + e;
                 const double ca = s_a[j], cb = s_b[j], cc = s_c[j];
                 const float2 v = s_v[j];
-                const double cn = cc * nx;
-                const float2 A = cis_f64(fma(ca, lx, cn));
-                const float2 B = cis_f64(fma(cb, lx, cn));
+                const float2 A = cis_f64(fma(ca, lcol, cc * ncol));
+                const float2 B = cis_f64(fma(cb, lrow, cc * nrow));
                 const float2 L = cmulf(v, B);
                 const float cf = (float)cc * cscale;
                 P9_SPLIT(P, L.x, Lrh, Lrl, e);
@@ -211,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
         if (wave == w) {
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int y = 8 * (r / 4) + 4 * grp + (r % 4), x = li;
+                const int y = ty * 32 + 8 * (r / 4) + 4 * grp + (r % 4), x = tx * 32 + li;
                 if (y < S && x < S) {
                     const float rr = (float)(kTwoPi * residual_n(x, y, S, a.image_size));
                     const float m1r = acc1r[r] * back1, m1i = acc1i[r] * back1;
@@ -359,6 +366,148 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
     }
 }
 
+
+// Degridder for S = 48, 64: rows and x span two 32-wide tiles.  The split
+// pixel operands (2 row tiles x 4 k-steps of 16) live in LDS, pre-arranged
+// in MFMA A-operand lane order; each wave takes tiles of 32 visibilities and
+// accumulates both row tiles in turn.
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void degrid_mfma_wide_kernel(KernelArgs a) {
+    static_assert(S > 32 && S <= 64, "two 32-wide tiles per axis");
+    typedef typename P::V V;
+    constexpr int RT = 2, KS = 4, NA = 6;  // row tiles, k-steps, operand arrays
+    __shared__ float2 s_tile[2][S * S];
+    __shared__ V s_op[RT * KS * NA * 64];
+    __shared__ double s_l[64], s_n1[64];
+    __shared__ float s_red[4];
+
+    const int sidx = a.order[blockIdx.x];
+    const SubgridGeom g = load_geom(a, sidx, S);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 31, grp = lane >> 5;
+    const int C = a.C, K = g.nt * C;
+    const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
+
+    splitter<S>(a, g, s_tile[0]);
+    if (tid < 64) {
+        s_l[tid] = (double)compute_l(tid, S, a.image_size);
+        s_n1[tid] = axis_n(tid, S, a.image_size);
+    }
+    __syncthreads();
+    const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, -1);
+
+    float pscale = 1.f;
+    if (P::kNormalise) {
+        float m = 0.f;
+        for (int p = tid; p < S * S; p += kThreads) m = fmaxf(m, fmaxf(fabsf(res[p].x), fabsf(res[p].y)));
+        pscale = 1.f / pow2_ceil(block_max(m, s_red));
+    }
+    const double r_corner = residual_n(0, 0, S, a.image_size);
+    const float rscale = 1.f / pow2_ceil((float)fabs(r_corner));
+
+    // operand (row tile rt, k-step h) for lane ln: row y = 32 rt + ln % 32,
+    // x = 16 h + 8 (ln / 32) + e; arrays Pr hi/lo, Pi hi/lo, r P re/im
+    for (int q = tid; q < RT * KS * 64; q += kThreads) {
+        const int ln = q & 63, h = (q >> 6) % KS, rt = q / (64 * KS);
+        const int y = rt * 32 + (ln & 31);
+        V prh, prl, pih, pil, qr, qi;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int x = 16 * h + 8 * (ln >> 5) + e;
+            float2 pv = make_float2(0.f, 0.f);
+            float rr = 0.f;
+            if (y < S && x < S) {
+                pv = res[((y + S / 2) % S) * S + (x + S / 2) % S];
+                pv.x *= pscale;
+                pv.y *= pscale;
+                rr = (float)residual_n(x, y, S, a.image_size) * rscale;
+            }
+            P9_SPLIT(P, pv.x, prh, prl, e);
+            P9_SPLIT(P, pv.y, pih, pil, e);
+            qr[e] = (typename P::T)(rr * pv.x);
+            qi[e] = (typename P::T)(rr * pv.y);
+        }
+        V* op = &s_op[((rt * KS + h) * NA) * 64 + ln];
+        op[0 * 64] = prh;
+        op[1 * 64] = prl;
+        op[2 * 64] = pih;
+        op[3 * 64] = pil;
+        op[4 * 64] = qr;
+        op[5 * 64] = qi;
+    }
+    __syncthreads();
+    const float out_scale = a.degrid_scale / pscale;
+    const float q1_back = kTwoPiF / rscale;
+
+    const int ntiles = (K + 31) / 32;
+    for (int vt = wave; vt < ntiles; vt += 4) {
+        const int j = vt * 32 + li;
+        const int jc = j < K ? j : K - 1;
+        const int t = g.t0 + jc / C, ch = jc % C;
+        const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
+        const double k = a.wavenumbers[ch];
+        const double ca = (g.u_off - k * p[0]) * kInvTwoPi;
+        const double cb = (g.v_off - k * p[1]) * kInvTwoPi;
+        const double cc = (g.w_off - k * p[2]) * kInvTwoPi;
+
+        V Arh[KS], Arl[KS], Aih[KS], Ail[KS];
+#pragma unroll
+        for (int h = 0; h < KS; h++) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int x = 16 * h + 8 * grp + e;
+                const float2 A = cis_f64(fma(ca, s_l[x], cc * s_n1[x]));
+                P9_SPLIT(P, A.x, Arh[h], Arl[h], e);
+                P9_SPLIT(P, A.y, Aih[h], Ail[h], e);
+            }
+        }
+        const float cf = (float)cc * q1_back;
+        float sr = 0.f, si = 0.f;
+#pragma unroll 1
+        for (int rt = 0; rt < RT; rt++) {
+            f32x16 q0r = {}, q0i = {}, q1r = {}, q1i = {};
+#pragma unroll
+            for (int h = 0; h < KS; h++) {
+                const V* op = &s_op[((rt * KS + h) * NA) * 64 + lane];
+                const V prh = op[0 * 64], prl = op[1 * 64], pih = op[2 * 64], pil = op[3 * 64];
+                const V qr = op[4 * 64], qi = op[5 * 64];
+                const V nAih = -Aih[h], nAil = -Ail[h];
+                // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai)
+                q0r = P::mma(prh, Arh[h], q0r);
+                q0i = P::mma(pih, Arh[h], q0i);
+                q1r = P::mma(qr, Arh[h], q1r);
+                q1i = P::mma(qi, Arh[h], q1i);
+                q0r = P::mma(prh, Arl[h], q0r);
+                q0i = P::mma(pih, Arl[h], q0i);
+                q1r = P::mma(qi, Aih[h], q1r);
+                q1i = P::mma(qr, nAih, q1i);
+                q0r = P::mma(prl, Arh[h], q0r);
+                q0i = P::mma(pil, Arh[h], q0i);
+                q0r = P::mma(pih, Aih[h], q0r);
+                q0i = P::mma(prh, nAih, q0i);
+                q0r = P::mma(pih, Ail[h], q0r);
+                q0i = P::mma(prh, nAil, q0i);
+                q0r = P::mma(pil, Aih[h], q0r);
+                q0i = P::mma(prl, nAih, q0i);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int y = rt * 32 + 8 * (r / 4) + 4 * grp + (r % 4);
+                if (y < S) {
+                    const float2 B = cis_f64(fma(cb, s_l[y], cc * s_n1[y]));
+                    const float zr = q0r[r] + cf * q1i[r];
+                    const float zi = q0i[r] - cf * q1r[r];
+                    sr = fmaf(B.x, zr, fmaf(B.y, zi, sr));
+                    si = fmaf(B.x, zi, fmaf(-B.y, zr, si));
+                }
+            }
+        }
+        sr += __shfl_xor(sr, 32);
+        si += __shfl_xor(si, 32);
+        if (grp == 0 && j < K) a.vis_out[vbase + j] = make_float2(sr * out_scale, si * out_scale);
+    }
+}
+
 }  // namespace dev
 
 #define P9_LAUNCH_MFMA(KERNEL, S, PREC) \
@@ -372,6 +521,8 @@ hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, hipStream
         case 16: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 16, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 16, dev::PrecBF16); break;
         case 24: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 24, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 24, dev::PrecBF16); break;
         case 32: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 32, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 32, dev::PrecBF16); break;
+        case 48: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 48, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 48, dev::PrecBF16); break;
+        case 64: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 64, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 64, dev::PrecBF16); break;
         default: return hipErrorNotSupported;
     }
     return hipGetLastError();
@@ -385,6 +536,8 @@ hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, hipStre
         case 16: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 16, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 16, dev::PrecBF16); break;
         case 24: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 24, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 24, dev::PrecBF16); break;
         case 32: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 32, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 32, dev::PrecBF16); break;
+        case 48: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 48, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 48, dev::PrecBF16); break;
+        case 64: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 64, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 64, dev::PrecBF16); break;
         default: return hipErrorNotSupported;
     }
     return hipGetLastError();

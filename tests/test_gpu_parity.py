@@ -106,8 +106,6 @@ CASES = [
 @pytest.mark.parametrize("S,K,L,wst,n_ant,T,C", CASES)
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_subgrid_sizes_and_wstack(O, S, K, L, wst, n_ant, T, C, kernel):
-    if kernel != "fp32" and S > 32:
-        pytest.skip("MFMA kernels cover subgrids up to 32")
     ob = obs.custom(G=256, S=S, K=K, L=L, n_ant=n_ant, T=T, C=C, seed=40 + S, core_diam=1500.0,
                     sources=[(1.0, 12 * obs.ARCSEC * 1.5, -9 * obs.ARCSEC * 1.5, 0.0), (0.3, -40 * obs.ARCSEC * 1.5, 5 * obs.ARCSEC * 1.5, 0.0)],
                     noise=0.01, wstep=None if wst else 0.0)
