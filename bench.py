@@ -136,6 +136,56 @@ def cpu_baseline_and_error(args, ob, part, model_grid_c64, kernel):
     return cpu, err
 
 
+def full_accuracy(ob, part, model_grid_c64, kernel):
+    """Max and RMS relative error of the dirty image, the PSF and the degridded
+    visibilities of the whole workload against the FP64 oracle (minutes of CPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import p9idg
+
+    O.set_threads(host_threads())
+    c = ob.c
+    oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize,
+                          c.MaxTimestepsPerSubgrid, float(c.ScaleArcSec), c.WLayerCount, c.WStep)
+    meta = part.subgrids
+    vis128 = ob.vis.astype(np.complex128)
+
+    def errs(a, b, mask=None):
+        d = np.abs(a.astype(np.float64) - b) if mask is None else np.abs(a - b)[mask]
+        r = np.abs(b) if mask is None else np.abs(b)[mask]
+        return {"max_rel": float(d.max() / r.max()), "rms_rel": float(np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(r ** 2)))}
+
+    out = {}
+    with p9idg.Plan(c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        dirty = plan.download_image()
+        plan.set_psf_vis(None)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        psf = plan.download_image()
+        plan.upload_model_grid(model_grid_c64)
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        vis = plan.download_model_vis()
+    t = time.perf_counter()
+    ref = O.gridw_to_image(oc, meta, vis128, ob.uvw, ob.freq)
+    out["dirty_image"] = errs(dirty, ref)
+    log("full accuracy: dirty image %.1f s" % (time.perf_counter() - t))
+    ref = O.gridw_to_image(oc, meta, np.ones_like(vis128), ob.uvw, ob.freq)
+    out["psf"] = errs(psf, ref)
+    log("full accuracy: psf done")
+    g = model_grid_c64.astype(np.complex128)
+    ref_v = O.degridw(oc, meta, g, ob.uvw, ob.freq)
+    covered = np.zeros(vis.shape[:2], dtype=bool)
+    for s in meta:
+        covered[s["baselineIdx"], s["timeSampleStart"]:s["timeSampleStart"] + s["timeSampleCount"]] = True
+    out["degridded_vis"] = errs(vis.astype(np.complex128), ref_v, covered)
+    log("full accuracy: degrid done")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +194,8 @@ def main():
     ap.add_argument("--kernel", default="fp32", choices=sorted(PEAK_TFLOPS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--full-accuracy", action="store_true",
+                    help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -216,13 +268,13 @@ def main():
         "frac": achieved / peak,
         "traffic": None,
     }
-    cpu, err = None, None
+    cpu, err, full = None, None, None
+    model_grid = plan.download_model_grid()
+    plan.close()
     if not args.no_cpu_baseline:
-        model_grid = plan.download_model_grid()
-        plan.close()
         cpu, err = cpu_baseline_and_error(args, ob, part, model_grid, args.kernel)
-    else:
-        plan.close()
+    if args.full_accuracy:
+        full = full_accuracy(ob, part, model_grid, args.kernel)
 
     out = {
         "metric": METRIC,
@@ -254,6 +306,8 @@ def main():
         "cpu_baseline": cpu,
         "error_vs_oracle": err,
     }
+    if full is not None:
+        out["error_vs_oracle_full"] = full
     print(json.dumps(out), flush=True)
 
 

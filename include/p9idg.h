@@ -132,6 +132,15 @@ int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan);
 int p9idg_destroy(p9idg_plan* plan);
 int p9idg_set_kernel(p9idg_plan* plan, int32_t kernel);
 int p9idg_get_kernel(const p9idg_plan* plan, int32_t* kernel);
+/* Launch shape of the gridder / degridder: queue != 0 runs persistent
+ * workgroups over a work queue whose items hold at most grid_chunk /
+ * degrid_chunk visibilities of one subgrid (0 = whole subgrids), with
+ * `workgroups` persistent workgroups (0 = CUs x occupancy); queue == 0
+ * launches one workgroup per subgrid.  Defaults: 1, 2048, 2048, 0. */
+int p9idg_set_launch(p9idg_plan* plan, int32_t queue, int32_t grid_chunk, int32_t degrid_chunk,
+                     int32_t workgroups);
+int p9idg_get_launch(const p9idg_plan* plan, int32_t* queue, int32_t* grid_chunk,
+                     int32_t* degrid_chunk, int32_t* workgroups);
 /* Uploads uvw, frequencies and the partition to HBM (once per observation). */
 int p9idg_set_observation(p9idg_plan* plan, const double* uvw, int32_t nr_baselines,
                           int32_t nr_timesteps, const double* freq, int32_t nr_channels,

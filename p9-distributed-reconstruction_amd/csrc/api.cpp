@@ -69,6 +69,19 @@ struct p9idg_plan {
     void* d_staging = nullptr;
     size_t staging_bytes = 0;
 
+    // work queues (one item list per direction) and the queue counter
+    std::vector<p9idg_subgrid> meta_host;
+    std::vector<int32_t> order_host;
+    WorkItem* d_items_grid = nullptr;
+    WorkItem* d_items_degrid = nullptr;
+    int32_t nitems_grid = 0, nitems_degrid = 0;
+    int32_t* d_counter = nullptr;
+    // launch shape (tunable): queue mode, visibilities per work item, workgroups
+    int32_t use_queue = 1;
+    int32_t grid_chunk = 2048;
+    int32_t degrid_chunk = 2048;
+    int32_t workgroups = 0;
+
     hipfftHandle fft_one = 0, fft_layers = 0;
     bool have_fft_one = false, have_fft_layers = false;
 
@@ -212,6 +225,59 @@ int span_end(p9idg_plan* p) {
     return P9IDG_OK;
 }
 
+// Splits every subgrid's visibilities into items of at most `chunk` (rounded
+// to multiples of 32) and orders them longest first.
+std::vector<WorkItem> make_items(const std::vector<p9idg_subgrid>& meta,
+                                 const std::vector<int32_t>& order, int C, int chunk) {
+    std::vector<WorkItem> items;
+    items.reserve(meta.size());
+    for (int32_t s : order) {
+        const int K = meta[s].time_sample_count * C;
+        if (K <= 0) {
+            items.push_back({s, 0, 0, 0});
+            continue;
+        }
+        const int n = chunk > 0 ? (K + chunk - 1) / chunk : 1;
+        int k0 = 0;
+        for (int i = 1; i <= n; i++) {
+            int k1 = i == n ? K : (int)(((int64_t)K * i / n + 31) / 32 * 32);
+            if (k1 > K) k1 = K;
+            if (k1 > k0) items.push_back({s, k0, k1, 0});
+            k0 = k1;
+        }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const WorkItem& a, const WorkItem& b) { return a.k1 - a.k0 > b.k1 - b.k0; });
+    return items;
+}
+
+int upload_items(p9idg_plan* p) {
+    DeviceGuard dg(p->device);
+    dev_free(p->d_items_grid);
+    dev_free(p->d_items_degrid);
+    p->nitems_grid = p->nitems_degrid = 0;
+    if (p->nsub == 0) return P9IDG_OK;
+    const auto gi = make_items(p->meta_host, p->order_host, p->C, p->grid_chunk);
+    const auto di = make_items(p->meta_host, p->order_host, p->C, p->degrid_chunk);
+    if (gi.size() > (size_t)INT32_MAX || di.size() > (size_t)INT32_MAX)
+        return fail(P9IDG_ERR_ARGUMENT, "too many work items");
+    if (int st = dev_alloc(&p->d_items_grid, gi.size())) return st;
+    if (int st = dev_alloc(&p->d_items_degrid, di.size())) return st;
+    P9_HIP(hipMemcpyAsync(p->d_items_grid, gi.data(), gi.size() * sizeof(WorkItem), hipMemcpyHostToDevice, p->stream));
+    P9_HIP(hipMemcpyAsync(p->d_items_degrid, di.data(), di.size() * sizeof(WorkItem), hipMemcpyHostToDevice, p->stream));
+    P9_HIP(hipStreamSynchronize(p->stream));
+    p->nitems_grid = (int32_t)gi.size();
+    p->nitems_degrid = (int32_t)di.size();
+    return P9IDG_OK;
+}
+
+LaunchCfg launch_cfg(const p9idg_plan* p) {
+    LaunchCfg cfg;
+    cfg.queue = p->use_queue != 0;
+    cfg.workgroups = p->workgroups;
+    return cfg;
+}
+
 KernelArgs make_args(const p9idg_plan* p, int mode) {
     KernelArgs a{};
     a.meta = p->d_meta;
@@ -234,6 +300,7 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
     const int S = p->c.subgrid_size;
     a.degrid_scale = 1.0f / (float)(S * S);                      // Gridder.cs:151
     a.mode = mode;
+    a.counter = p->d_counter;
     return a;
 }
 
@@ -252,15 +319,17 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     }
     KernelArgs a = make_args(p, mode);
     a.grid = target;
+    a.items = p->d_items_grid;
+    a.nitems = p->nitems_grid;
     P9_HIP(hipEventRecord(p->ev[0], p->stream));
     P9_HIP(launch_fill_zero(target, n, p->stream));
     P9_HIP(hipEventRecord(p->ev[1], p->stream));
     if (int st = span_begin(p, 0)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
-        e = launch_grid_fp32(a, p->c.subgrid_size, p->stream);
+        e = launch_grid_fp32(a, p->c.subgrid_size, launch_cfg(p), p->stream);
     else
-        e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, p->stream);
+        e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p), p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
     if (int st = span_end(p)) return st;
     P9_HIP(hipEventRecord(p->ev[2], p->stream));
@@ -274,14 +343,16 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
         return fail(P9IDG_ERR_ARGUMENT, "unknown degrid mode");
     DeviceGuard dg(p->device);
     KernelArgs a = make_args(p, mode);
+    a.items = p->d_items_degrid;
+    a.nitems = p->nitems_degrid;
     P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));  // Gridder.cs:96
     P9_HIP(hipEventRecord(p->ev[3], p->stream));
     if (int st = span_begin(p, 1)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
-        e = launch_degrid_fp32(a, p->c.subgrid_size, p->stream);
+        e = launch_degrid_fp32(a, p->c.subgrid_size, launch_cfg(p), p->stream);
     else
-        e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, p->stream);
+        e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p), p->stream);
     if (e != hipSuccess) return hip_fail(e, "degridder launch");
     if (int st = span_end(p)) return st;
     P9_HIP(hipEventRecord(p->ev[4], p->stream));
@@ -455,6 +526,7 @@ int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
     if ((st = dev_alloc(&p->d_grid, (size_t)G * G))) return bail(st);
     if ((st = dev_alloc(&p->d_model_grid, (size_t)G * G))) return bail(st);
     if ((st = dev_alloc(&p->d_image, (size_t)G * G))) return bail(st);
+    if ((st = dev_alloc(&p->d_counter, (size_t)1))) return bail(st);
     e = hipMemcpy(p->d_twiddle, tw.data(), S * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(hip_fail(e, "twiddle upload"));
     e = hipMemcpy(p->d_phasor, ph.data(), ph.size() * sizeof(float2), hipMemcpyHostToDevice);
@@ -483,6 +555,9 @@ int p9idg_destroy(p9idg_plan* p) {
     dev_free(p->d_twiddle);
     dev_free(p->d_phasor);
     dev_free(p->d_flags);
+    dev_free(p->d_counter);
+    dev_free(p->d_items_grid);
+    dev_free(p->d_items_degrid);
     if (p->d_staging) (void)hipFree(p->d_staging);
     for (auto& ev : p->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -505,6 +580,29 @@ int p9idg_get_kernel(const p9idg_plan* p, int32_t* kernel) {
     if (int st = check_plan(p)) return st;
     if (!kernel) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
     *kernel = p->kernel;
+    return P9IDG_OK;
+}
+
+int p9idg_set_launch(p9idg_plan* p, int32_t queue, int32_t grid_chunk, int32_t degrid_chunk,
+                     int32_t workgroups) {
+    if (int st = check_plan(p)) return st;
+    if (grid_chunk < 0 || degrid_chunk < 0 || workgroups < 0)
+        return fail(P9IDG_ERR_ARGUMENT, "negative launch parameter");
+    p->use_queue = queue != 0;
+    p->grid_chunk = grid_chunk;
+    p->degrid_chunk = degrid_chunk;
+    p->workgroups = workgroups;
+    if (p->have_obs) return upload_items(p);
+    return P9IDG_OK;
+}
+
+int p9idg_get_launch(const p9idg_plan* p, int32_t* queue, int32_t* grid_chunk,
+                     int32_t* degrid_chunk, int32_t* workgroups) {
+    if (int st = check_plan(p)) return st;
+    if (queue) *queue = p->use_queue;
+    if (grid_chunk) *grid_chunk = p->grid_chunk;
+    if (degrid_chunk) *degrid_chunk = p->degrid_chunk;
+    if (workgroups) *workgroups = p->workgroups;
     return P9IDG_OK;
 }
 
@@ -575,6 +673,9 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
     p->nsub = nsub;
     p->plain_adder_ok = plain_ok;
     p->gridded_vis = gridded;
+    p->meta_host.assign(meta, meta + nsub);
+    p->order_host = order;
+    if (int st = upload_items(p)) return st;
     p->have_obs = true;
     return P9IDG_OK;
 }

@@ -24,8 +24,9 @@ __device__ __forceinline__ float compute_l(int x, int S, float image_size) {
     t = __fmul_rn(t, image_size);
     return __fdiv_rn(t, (float)S);
 }
-// Gridder.cs:168-174: n = 1 - sqrt(1 - l^2 - m^2) written to keep precision
-// for small l, m; the sqrt is taken in double.
+// Gridder.cs:168-174.  The reference evaluates tmp / 1.0f + Sqrt(1.0 - tmp),
+// so n = t / (t + sqrt(1 - t)) with t = l^2 + m^2 (about twice
+// 1 - sqrt(1 - t) for small t); kept as written.  The sqrt is taken in double.
 __device__ __forceinline__ float compute_n(float l, float m) {
     float tmp = __fadd_rn(__fmul_rn(l, l), __fmul_rn(m, m));
     double d = (double)tmp + sqrt(1.0 - (double)tmp);
@@ -198,6 +199,23 @@ __device__ __forceinline__ float2* fft_2d(float2* a, float2* b, const float2* tw
     fft_1d_all<S>(src, dst, tw, sign, S, 1);  // rows
     fft_1d_all<S>(src, dst, tw, sign, 1, S);  // columns
     return src;
+}
+
+// Persistent work queue: each workgroup takes the next WorkItem from the
+// counter until every item is taken; body(sidx, k0, k1) processes one.  All
+// threads of the workgroup leave together once the counter passes nitems.
+template <class Body>
+__device__ __forceinline__ void run_queue(const KernelArgs& a, Body body) {
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();  // previous item's LDS reads are done
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= a.nitems) return;
+        const WorkItem w = a.items[it];
+        body(w.sidx, w.k0, w.k1);
+    }
 }
 
 // Index of grid row/column r after FFT.Shift (quadrant swap) for even G.

@@ -15,15 +15,15 @@
 namespace p9idg {
 namespace dev {
 
+// Visibilities [k0, k1) of subgrid s (k1 < 0: all of them).
 template <int S>
-__global__ __launch_bounds__(kThreads) void grid_fp32_kernel(KernelArgs a) {
+__device__ __forceinline__ void grid_fp32_body(const KernelArgs& a, int s, int k0, int k1) {
     constexpr int NPIX = (S * S + kThreads - 1) / kThreads;
     constexpr int CHUNK = kThreads;
     __shared__ float4 s_coef[CHUNK];
     __shared__ float2 s_vis[CHUNK];
     __shared__ float2 s_tile[2][S * S];
 
-    const int s = a.order[blockIdx.x];
     const SubgridGeom g = load_geom(a, s, S);
     const int tid = threadIdx.x;
 
@@ -40,9 +40,9 @@ __global__ __launch_bounds__(kThreads) void grid_fp32_kernel(KernelArgs a) {
     }
 
     const int C = a.C;
-    const int K = g.nt * C;
+    const int K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
-    for (int j0 = 0; j0 < K; j0 += CHUNK) {
+    for (int j0 = k0; j0 < K; j0 += CHUNK) {
         const int j = j0 + tid;
         if (j < K) {
             const int t = g.t0 + j / C, ch = j % C;
@@ -82,12 +82,11 @@ __global__ __launch_bounds__(kThreads) void grid_fp32_kernel(KernelArgs a) {
 }
 
 template <int S>
-__global__ __launch_bounds__(kThreads) void degrid_fp32_kernel(KernelArgs a) {
+__device__ __forceinline__ void degrid_fp32_body(const KernelArgs& a, int s, int k0, int k1) {
     __shared__ float2 s_tile[2][S * S];
     __shared__ float s_n[S * S];
     __shared__ float s_l[S];
 
-    const int s = a.order[blockIdx.x];
     const SubgridGeom g = load_geom(a, s, S);
     const int tid = threadIdx.x;
 
@@ -108,9 +107,9 @@ __global__ __launch_bounds__(kThreads) void degrid_fp32_kernel(KernelArgs a) {
     __syncthreads();
 
     const int C = a.C;
-    const int K = g.nt * C;
+    const int K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
-    for (int j = tid; j < K; j += kThreads) {
+    for (int j = k0 + tid; j < K; j += kThreads) {
         const int t = g.t0 + j / C, ch = j % C;
         const float3 cf = vis_coeffs(a, g, t, ch);
         float sx = 0.f, sy = 0.f;
@@ -131,28 +130,72 @@ __global__ __launch_bounds__(kThreads) void degrid_fp32_kernel(KernelArgs a) {
     }
 }
 
+template <int S>
+__global__ __launch_bounds__(kThreads) void grid_fp32_kernel(KernelArgs a) {
+    grid_fp32_body<S>(a, a.order[blockIdx.x], 0, -1);
+}
+template <int S>
+__global__ __launch_bounds__(kThreads) void grid_fp32_queue_kernel(KernelArgs a) {
+    run_queue(a, [&](int s, int k0, int k1) { grid_fp32_body<S>(a, s, k0, k1); });
+}
+template <int S>
+__global__ __launch_bounds__(kThreads) void degrid_fp32_kernel(KernelArgs a) {
+    degrid_fp32_body<S>(a, a.order[blockIdx.x], 0, -1);
+}
+template <int S>
+__global__ __launch_bounds__(kThreads) void degrid_fp32_queue_kernel(KernelArgs a) {
+    run_queue(a, [&](int s, int k0, int k1) { degrid_fp32_body<S>(a, s, k0, k1); });
+}
+
+__global__ void reset_counter_kernel(int32_t* counter) { *counter = 0; }
+
 }  // namespace dev
 
-#define P9_DISPATCH_S(S, KERNEL)                                                          \
-    switch (S) {                                                                          \
-        case 8: hipLaunchKernelGGL(KERNEL<8>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); break;   \
-        case 16: hipLaunchKernelGGL(KERNEL<16>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); break; \
-        case 24: hipLaunchKernelGGL(KERNEL<24>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); break; \
-        case 32: hipLaunchKernelGGL(KERNEL<32>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); break; \
-        case 48: hipLaunchKernelGGL(KERNEL<48>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); break; \
-        case 64: hipLaunchKernelGGL(KERNEL<64>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); break; \
-        default: return hipErrorInvalidValue;                                             \
+// Launches KERNEL<S> (one workgroup per subgrid) or QKERNEL<S> (persistent
+// queue over the work items).
+#define P9_LAUNCH_FP32(KERNEL, QKERNEL, SV)                                                   \
+    do {                                                                                      \
+        if (cfg.queue) {                                                                      \
+            int wg = cfg.workgroups > 0 ? cfg.workgroups                                      \
+                                        : persistent_workgroups((const void*)QKERNEL<SV>, dev::kThreads); \
+            wg = wg < a.nitems ? wg : a.nitems;                                               \
+            if (wg <= 0) return hipSuccess;                                                   \
+            hipLaunchKernelGGL(dev::reset_counter_kernel, dim3(1), dim3(1), 0, st, a.counter); \
+            hipLaunchKernelGGL(QKERNEL<SV>, dim3(wg), dim3(dev::kThreads), 0, st, a);         \
+        } else {                                                                              \
+            hipLaunchKernelGGL(KERNEL<SV>, dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); \
+        }                                                                                     \
+    } while (0)
+
+#define P9_SWITCH_S(S, BODY_FOR)                  \
+    switch (S) {                                  \
+        case 8: BODY_FOR(8); break;               \
+        case 16: BODY_FOR(16); break;             \
+        case 24: BODY_FOR(24); break;             \
+        case 32: BODY_FOR(32); break;             \
+        case 48: BODY_FOR(48); break;             \
+        case 64: BODY_FOR(64); break;             \
+        default: return hipErrorInvalidValue;     \
     }
 
-hipError_t launch_grid_fp32(const KernelArgs& a, int S, hipStream_t st) {
+hipError_t launch_grid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st) {
     if (a.nr_subgrids <= 0) return hipSuccess;
-    P9_DISPATCH_S(S, dev::grid_fp32_kernel);
+#define P9_G(SV) P9_LAUNCH_FP32(dev::grid_fp32_kernel, dev::grid_fp32_queue_kernel, SV)
+    P9_SWITCH_S(S, P9_G)
+#undef P9_G
     return hipGetLastError();
 }
 
-hipError_t launch_degrid_fp32(const KernelArgs& a, int S, hipStream_t st) {
+hipError_t launch_degrid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st) {
     if (a.nr_subgrids <= 0) return hipSuccess;
-    P9_DISPATCH_S(S, dev::degrid_fp32_kernel);
+#define P9_D(SV) P9_LAUNCH_FP32(dev::degrid_fp32_kernel, dev::degrid_fp32_queue_kernel, SV)
+    P9_SWITCH_S(S, P9_D)
+#undef P9_D
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counter(int32_t* counter, hipStream_t st) {
+    hipLaunchKernelGGL(dev::reset_counter_kernel, dim3(1), dim3(1), 0, st, counter);
     return hipGetLastError();
 }
 

@@ -104,7 +104,7 @@ constexpr float kTwoPiF = 6.2831853071795864769f;
 // for S = 48, 64 the subgrid is 2 x 2 tiles and wave w owns tile w over all
 // visibilities.  Partial subgrids are summed in LDS.
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
+__device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
     constexpr int NT = (S + 31) / 32;  // tiles per axis
     constexpr int WPK = 4 / (NT * NT); // waves sharing one tile's visibilities
     static_assert(NT * 32 >= S && NT <= 2, "subgrids up to 64");
@@ -116,11 +116,10 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
     __shared__ float2 s_tile[2][S * S];
     __shared__ float s_red[4];
 
-    const int sidx = a.order[blockIdx.x];
     const SubgridGeom g = load_geom(a, sidx, S);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
-    const int C = a.C, K = g.nt * C;
+    const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
 
     for (int p = tid; p < S * S; p += kThreads) s_tile[0][p] = make_float2(0.f, 0.f);
@@ -131,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
     float cscale = 1.f;
     if (P::kNormalise) {
         float m = 0.f;
-        for (int j = tid; j < K; j += kThreads) {
+        for (int j = k0 + tid; j < K; j += kThreads) {
             const float2 v = a.vis[vbase + j];
             m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
         }
@@ -149,7 +148,7 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
 
     f32x16 acc0r = {}, acc0i = {}, acc1r = {}, acc1i = {};
 
-    for (int j0 = 0; j0 < K; j0 += ROUND) {
+    for (int j0 = k0; j0 < K; j0 += ROUND) {
         {
             const int j = j0 + tid;
             if (j < K) {
@@ -241,18 +240,17 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
 // subgrid pixels split into registers as the MFMA A operand (rows y, k = x),
 // then each wave takes tiles of 32 visibilities (the N dimension).
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
+__device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
     static_assert(S <= 32, "one 32x32 MFMA tile per subgrid");
     typedef typename P::V V;
     __shared__ float2 s_tile[2][S * S];
     __shared__ double s_l[32], s_n1[32];
     __shared__ float s_red[4];
 
-    const int sidx = a.order[blockIdx.x];
     const SubgridGeom g = load_geom(a, sidx, S);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
-    const int C = a.C, K = g.nt * C;
+    const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
 
     splitter<S>(a, g, s_tile[0]);
@@ -302,9 +300,9 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
     const float out_scale = a.degrid_scale / pscale;
     const float q1_back = kTwoPiF / rscale;
 
-    const int ntiles = (K + 31) / 32;
+    const int ntiles = (K - k0 + 31) / 32;
     for (int vt = wave; vt < ntiles; vt += 4) {
-        const int j = vt * 32 + li;
+        const int j = k0 + vt * 32 + li;
         const int jc = j < K ? j : K - 1;
         const int t = g.t0 + jc / C, ch = jc % C;
         const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
@@ -372,7 +370,7 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
 // in MFMA A-operand lane order; each wave takes tiles of 32 visibilities and
 // accumulates both row tiles in turn.
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void degrid_mfma_wide_kernel(KernelArgs a) {
+__device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int sidx, int k0, int k1) {
     static_assert(S > 32 && S <= 64, "two 32-wide tiles per axis");
     typedef typename P::V V;
     constexpr int RT = 2, KS = 4, NA = 6;  // row tiles, k-steps, operand arrays
@@ -381,11 +379,10 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_wide_kernel(KernelArgs a
     __shared__ double s_l[64], s_n1[64];
     __shared__ float s_red[4];
 
-    const int sidx = a.order[blockIdx.x];
     const SubgridGeom g = load_geom(a, sidx, S);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
-    const int C = a.C, K = g.nt * C;
+    const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
 
     splitter<S>(a, g, s_tile[0]);
@@ -439,9 +436,9 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_wide_kernel(KernelArgs a
     const float out_scale = a.degrid_scale / pscale;
     const float q1_back = kTwoPiF / rscale;
 
-    const int ntiles = (K + 31) / 32;
+    const int ntiles = (K - k0 + 31) / 32;
     for (int vt = wave; vt < ntiles; vt += 4) {
-        const int j = vt * 32 + li;
+        const int j = k0 + vt * 32 + li;
         const int jc = j < K ? j : K - 1;
         const int t = g.t0 + jc / C, ch = jc % C;
         const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
@@ -508,38 +505,74 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_wide_kernel(KernelArgs a
     }
 }
 
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
+    grid_mfma_body<S, P>(a, a.order[blockIdx.x], 0, -1);
+}
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void grid_mfma_queue_kernel(KernelArgs a) {
+    run_queue(a, [&](int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
+}
+
+// S <= 32: one tile; S = 48, 64: the wide form
+template <int S, class P>
+__device__ __forceinline__ void degrid_mfma_any(const KernelArgs& a, int s, int k0, int k1) {
+    if constexpr (S <= 32)
+        degrid_mfma_body<S, P>(a, s, k0, k1);
+    else
+        degrid_mfma_wide_body<S, P>(a, s, k0, k1);
+}
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
+    degrid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1);
+}
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void degrid_mfma_queue_kernel(KernelArgs a) {
+    run_queue(a, [&](int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
+}
+
 }  // namespace dev
 
-#define P9_LAUNCH_MFMA(KERNEL, S, PREC) \
-    hipLaunchKernelGGL((KERNEL<S, PREC>), dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a)
+#define P9_LAUNCH_MFMA(KERNEL, QKERNEL, SV, PREC)                                                  \
+    do {                                                                                           \
+        if (cfg.queue) {                                                                           \
+            int wg = cfg.workgroups > 0                                                            \
+                         ? cfg.workgroups                                                          \
+                         : persistent_workgroups((const void*)QKERNEL<SV, PREC>, dev::kThreads);   \
+            wg = wg < a.nitems ? wg : a.nitems;                                                    \
+            if (wg <= 0) return hipSuccess;                                                        \
+            hipError_t e_ = launch_reset_counter(a.counter, st);                                   \
+            if (e_ != hipSuccess) return e_;                                                       \
+            hipLaunchKernelGGL((QKERNEL<SV, PREC>), dim3(wg), dim3(dev::kThreads), 0, st, a);      \
+        } else {                                                                                   \
+            hipLaunchKernelGGL((KERNEL<SV, PREC>), dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); \
+        }                                                                                          \
+    } while (0)
 
-hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st) {
+#define P9_MFMA_SWITCH(KERNEL, QKERNEL)                                              \
+    switch (S) {                                                                     \
+        case 8: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 8, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 8, dev::PrecBF16); break;    \
+        case 16: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 16, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 16, dev::PrecBF16); break; \
+        case 24: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 24, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 24, dev::PrecBF16); break; \
+        case 32: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 32, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 32, dev::PrecBF16); break; \
+        case 48: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 48, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 48, dev::PrecBF16); break; \
+        case 64: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 64, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 64, dev::PrecBF16); break; \
+        default: return hipErrorNotSupported;                                        \
+    }
+
+hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
+                            hipStream_t st) {
     if (a.nr_subgrids <= 0) return hipSuccess;
     const bool f16 = precision == P9IDG_KERNEL_MFMA_FP16;
-    switch (S) {
-        case 8: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 8, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 8, dev::PrecBF16); break;
-        case 16: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 16, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 16, dev::PrecBF16); break;
-        case 24: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 24, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 24, dev::PrecBF16); break;
-        case 32: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 32, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 32, dev::PrecBF16); break;
-        case 48: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 48, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 48, dev::PrecBF16); break;
-        case 64: if (f16) P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 64, dev::PrecF16); else P9_LAUNCH_MFMA(dev::grid_mfma_kernel, 64, dev::PrecBF16); break;
-        default: return hipErrorNotSupported;
-    }
+    P9_MFMA_SWITCH(dev::grid_mfma_kernel, dev::grid_mfma_queue_kernel)
     return hipGetLastError();
 }
 
-hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st) {
+hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
+                              hipStream_t st) {
     if (a.nr_subgrids <= 0) return hipSuccess;
     const bool f16 = precision == P9IDG_KERNEL_MFMA_FP16;
-    switch (S) {
-        case 8: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 8, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 8, dev::PrecBF16); break;
-        case 16: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 16, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 16, dev::PrecBF16); break;
-        case 24: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 24, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 24, dev::PrecBF16); break;
-        case 32: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 32, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_kernel, 32, dev::PrecBF16); break;
-        case 48: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 48, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 48, dev::PrecBF16); break;
-        case 64: if (f16) P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 64, dev::PrecF16); else P9_LAUNCH_MFMA(dev::degrid_mfma_wide_kernel, 64, dev::PrecBF16); break;
-        default: return hipErrorNotSupported;
-    }
+    P9_MFMA_SWITCH(dev::degrid_mfma_kernel, dev::degrid_mfma_queue_kernel)
     return hipGetLastError();
 }
 

@@ -1,6 +1,9 @@
 // Element-wise HBM-bound kernels around the gridder: buffer fills, visibility
 // conversion, the PSF visibilities, and the image <-> grid steps around the
 // G x G inverse / forward FFTs (FFT.cs:99-117, :159-215, :248-264).
+#include <map>
+#include <mutex>
+
 #include "device_common.h"
 
 namespace p9idg {
@@ -86,6 +89,23 @@ inline unsigned blocks_for(size_t n) {
 }
 
 }  // namespace dev
+
+int persistent_workgroups(const void* kernel, int threads) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_pair(kernel, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 1;
+    const int n = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    cache[key] = n;
+    return n;
+}
 
 hipError_t launch_fill_zero(float2* p, size_t n, hipStream_t st) {
     // n complex64 values; buffers are allocated in multiples of 2 complex.

@@ -42,6 +42,20 @@ int partition_host(const p9idg_constants& c, const double* uvw, int B, int T, co
 
 bool subgrid_size_supported(int s);
 
+// One unit of gridder / degridder work: visibilities [k0, k1) of subgrid
+// `sidx` (k counts the subgrid's visibilities in time-major, channel-minor
+// order).  Partial subgrids of one subgrid add linearly through the subgrid
+// FFT and the adder, so large subgrids can be split across items.
+struct WorkItem {
+    int32_t sidx, k0, k1, inst;
+};
+
+// How a gridder / degridder launch is shaped.
+struct LaunchCfg {
+    bool queue = true;      // persistent workgroups pulling WorkItems from a counter
+    int32_t workgroups = 0; // persistent workgroups (0: device CUs x occupancy)
+};
+
 // Kernel launch parameters shared by host and device code.
 struct KernelArgs {
     const p9idg_subgrid* meta;
@@ -61,13 +75,21 @@ struct KernelArgs {
     double w_offset_scale; // 2 pi * WStepLambda (0 in the reference)
     float degrid_scale;    // 1.0f / (S * S)  (Gridder.cs:151)
     int32_t mode;
+    const WorkItem* items; // work queue (longest first)
+    int32_t nitems;
+    int32_t* counter;      // next item; zeroed by the launcher before each launch
 };
 
 // Launchers (kernels_*.hip); S is the subgrid size.
-hipError_t launch_grid_fp32(const KernelArgs& a, int S, hipStream_t st);
-hipError_t launch_degrid_fp32(const KernelArgs& a, int S, hipStream_t st);
-hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st);
-hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, hipStream_t st);
+hipError_t launch_grid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st);
+hipError_t launch_degrid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st);
+hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
+                            hipStream_t st);
+hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
+                              hipStream_t st);
+hipError_t launch_reset_counter(int32_t* counter, hipStream_t st);
+// Persistent workgroups for a kernel: CUs x resident workgroups per CU.
+int persistent_workgroups(const void* kernel, int threads);
 hipError_t launch_fill_zero(float2* p, size_t n, hipStream_t st);
 hipError_t launch_c128_to_c64(const double2* in, float2* out, size_t n, hipStream_t st);
 hipError_t launch_psf_vis(const uint8_t* flags, float2* out, size_t n, hipStream_t st);

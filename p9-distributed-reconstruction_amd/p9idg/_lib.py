@@ -94,6 +94,8 @@ IDG_SIGNATURES = {
     "p9idg_set_kernel": [_vp, _i32],
     "p9idg_get_kernel": [_vp, ctypes.POINTER(_i32)],
     "p9idg_set_observation": [_vp, _vp, _i32, _i32, _vp, _i32, _vp, _i64],
+    "p9idg_set_launch": [_vp, _i32, _i32, _i32, _i32],
+    "p9idg_get_launch": [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     "p9idg_grid": [_vp, _vp, _i32, _vp],
     "p9idg_degrid": [_vp, _vp, _vp],
     "p9idg_gridw": [_vp, _vp, _i32, _vp],

@@ -184,6 +184,14 @@ class Plan:
         k = _KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
         check(_lib.idg().p9idg_set_kernel(self._h, k))
 
+    def set_launch(self, queue=True, grid_chunk=2048, degrid_chunk=2048, workgroups=0):
+        check(_lib.idg().p9idg_set_launch(self._h, 1 if queue else 0, int(grid_chunk), int(degrid_chunk), int(workgroups)))
+
+    def get_launch(self):
+        v = [ctypes.c_int32() for _ in range(4)]
+        check(_lib.idg().p9idg_get_launch(self._h, *[ctypes.byref(x) for x in v]))
+        return {"queue": bool(v[0].value), "grid_chunk": v[1].value, "degrid_chunk": v[2].value, "workgroups": v[3].value}
+
     def set_observation(self, uvw, frequencies, metadata):
         uvw = np.ascontiguousarray(uvw, dtype=np.float64)
         freq = np.ascontiguousarray(frequencies, dtype=np.float64)
