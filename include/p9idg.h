@@ -113,6 +113,12 @@ typedef enum p9idg_degrid_mode {
     P9IDG_DEGRID_WSTACK = 1 /* AdderWStack.Split (IDG.DeGridW), AdderWStack.cs:81 */
 } p9idg_degrid_mode;
 
+/* Input of p9idg_exec_grid. */
+typedef enum p9idg_grid_source {
+    P9IDG_SRC_VIS = 0,     /* the uploaded visibilities */
+    P9IDG_SRC_RESIDUAL = 1 /* the residuals of p9idg_exec_residual */
+} p9idg_grid_source;
+
 typedef struct p9idg_plan p9idg_plan;
 
 int p9idg_api_version(void);
@@ -200,6 +206,21 @@ int p9idg_exec_image_to_grid(p9idg_plan* plan);
  * fused launch) into the plan's complex64 model-visibility buffer. */
 int p9idg_exec_degrid(p9idg_plan* plan, int32_t mode);
 int p9idg_sync(p9idg_plan* plan);
+/* Flags uint8[B][T][C] (nonzero = flagged) used by the residuals; NULL clears. */
+int p9idg_set_flags(p9idg_plan* plan, const uint8_t* flags);
+/* Visibilities.Substract (Common.cs:305-318) on the device:
+ * residual = vis - model_vis, 0 where flagged. */
+int p9idg_exec_residual(p9idg_plan* plan);
+int p9idg_set_grid_source(p9idg_plan* plan, int32_t src);
+/* One device step of the major cycle (MajorCycle.cs:125-129, then :88-90):
+ * ToUVW of the plan's image -> DeGridW -> residuals -> grid the residuals
+ * (grid_mode) -> inverse FFT -> image (replacing the model image with the
+ * dirty image of the residuals).  exec runs it eagerly; capture records it
+ * into a hipGraph (after one eager run) that launch replays on the stream. */
+int p9idg_exec_cycle(p9idg_plan* plan, int32_t grid_mode);
+int p9idg_capture_cycle(p9idg_plan* plan, int32_t grid_mode);
+int p9idg_launch_cycle(p9idg_plan* plan);
+int p9idg_download_residual(p9idg_plan* plan, void* out_c64);
 
 /* Device pointers of the plan's buffers (for zero-copy interop). */
 int p9idg_device_buffers(p9idg_plan* plan, void** vis, void** grid, void** model_grid,

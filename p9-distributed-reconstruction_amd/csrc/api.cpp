@@ -77,6 +77,15 @@ struct p9idg_plan {
     int32_t nitems_grid = 0, nitems_degrid = 0;
     int32_t* d_counter = nullptr;
     // launch shape (tunable): queue mode, visibilities per work item, workgroups
+    // residual visibilities (vis - model_vis, 0 where flagged) and the
+    // gridder's input selector
+    float2* d_resid = nullptr;
+    bool have_flags = false;
+    int32_t grid_src = P9IDG_SRC_VIS;
+    // captured major-cycle step
+    bool capturing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     int32_t use_queue = 1;
     int32_t grid_chunk = 2048;
     int32_t degrid_chunk = 2048;
@@ -209,9 +218,16 @@ hipEvent_t pooled_event(p9idg_plan* p, size_t i) {
     return p->event_pool[i];
 }
 
+// Records a plan event unless the stream is being captured into a graph.
+int record(p9idg_plan* p, int i) {
+    if (p->capturing) return P9IDG_OK;
+    P9_HIP(hipEventRecord(p->ev[i], p->stream));
+    return P9IDG_OK;
+}
+
 // Starts a timed span on the plan's stream when collection is on.
 int span_begin(p9idg_plan* p, int kind) {
-    if (!p->collect) return P9IDG_OK;
+    if (!p->collect || p->capturing) return P9IDG_OK;
     const size_t i = p->spans.size() * 2;
     hipEvent_t a = pooled_event(p, i), b = pooled_event(p, i + 1);
     if (!a || !b) return fail(P9IDG_ERR_HIP, "event creation failed");
@@ -220,7 +236,7 @@ int span_begin(p9idg_plan* p, int kind) {
     return P9IDG_OK;
 }
 int span_end(p9idg_plan* p) {
-    if (!p->collect || p->spans.empty()) return P9IDG_OK;
+    if (!p->collect || p->capturing || p->spans.empty()) return P9IDG_OK;
     P9_HIP(hipEventRecord(p->spans.back().stop, p->stream));
     return P9IDG_OK;
 }
@@ -285,7 +301,7 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
     a.nr_subgrids = (int32_t)p->nsub;
     a.uvw = p->d_uvw;
     a.wavenumbers = p->d_wavenumbers;
-    a.vis = p->d_vis;
+    a.vis = p->grid_src == P9IDG_SRC_RESIDUAL ? p->d_resid : p->d_vis;
     a.vis_out = p->d_model_vis;
     a.twiddle = p->d_twiddle;
     a.adder_phasor = p->d_phasor;
@@ -309,6 +325,8 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     if (mode < P9IDG_GRID_PLAIN || mode > P9IDG_GRID_WSUM)
         return fail(P9IDG_ERR_ARGUMENT, "unknown grid mode");
     if (mode == P9IDG_GRID_PLAIN && !p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    if (p->grid_src == P9IDG_SRC_RESIDUAL && !p->d_resid)
+        return fail(P9IDG_ERR_STATE, "no residual visibilities (p9idg_exec_residual)");
     DeviceGuard dg(p->device);
     float2* target = p->d_grid;
     size_t n = grid_elems(p);
@@ -321,9 +339,9 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     a.grid = target;
     a.items = p->d_items_grid;
     a.nitems = p->nitems_grid;
-    P9_HIP(hipEventRecord(p->ev[0], p->stream));
+    if (int st_ = record(p, 0)) return st_;
     P9_HIP(launch_fill_zero(target, n, p->stream));
-    P9_HIP(hipEventRecord(p->ev[1], p->stream));
+    if (int st_ = record(p, 1)) return st_;
     if (int st = span_begin(p, 0)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
@@ -332,7 +350,7 @@ int do_exec_grid(p9idg_plan* p, int mode) {
         e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p), p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
     if (int st = span_end(p)) return st;
-    P9_HIP(hipEventRecord(p->ev[2], p->stream));
+    if (int st_ = record(p, 2)) return st_;
     p->timed_grid = true;
     return P9IDG_OK;
 }
@@ -346,7 +364,7 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     a.items = p->d_items_degrid;
     a.nitems = p->nitems_degrid;
     P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));  // Gridder.cs:96
-    P9_HIP(hipEventRecord(p->ev[3], p->stream));
+    if (int st_ = record(p, 3)) return st_;
     if (int st = span_begin(p, 1)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
@@ -355,7 +373,7 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
         e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p), p->stream);
     if (e != hipSuccess) return hip_fail(e, "degridder launch");
     if (int st = span_end(p)) return st;
-    P9_HIP(hipEventRecord(p->ev[4], p->stream));
+    if (int st_ = record(p, 4)) return st_;
     p->timed_degrid = true;
     return P9IDG_OK;
 }
@@ -366,7 +384,7 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
     DeviceGuard dg(p->device);
     const int G = p->c.grid_size;
     const double inv_n = 1.0 / (double)p->c.visibilities_count;
-    P9_HIP(hipEventRecord(p->ev[5], p->stream));
+    if (int st_ = record(p, 5)) return st_;
     if (mode == P9IDG_GRID_WSTACK) {
         if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
         if (int st = ensure_fft_layers(p)) return st;
@@ -380,7 +398,7 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
                              reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
         P9_HIP(launch_layers_to_image(p->d_grid, 1, G, inv_n, 1, p->d_image, p->stream));
     }
-    P9_HIP(hipEventRecord(p->ev[6], p->stream));
+    if (int st_ = record(p, 6)) return st_;
     p->timed_fft = true;
     return P9IDG_OK;
 }
@@ -393,6 +411,36 @@ int do_image_to_grid(p9idg_plan* p) {
     P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_model_grid),
                          reinterpret_cast<hipfftComplex*>(p->d_model_grid), HIPFFT_FORWARD));
     return P9IDG_OK;
+}
+
+int do_residual(p9idg_plan* p) {
+    if (int st = check_obs(p)) return st;
+    DeviceGuard dg(p->device);
+    if (int st = dev_alloc(&p->d_resid, vis_elems(p))) return st;
+    P9_HIP(launch_residual(p->d_vis, p->d_model_vis, p->have_flags ? p->d_flags : nullptr, p->d_resid,
+                           vis_elems(p), p->stream));
+    return P9IDG_OK;
+}
+
+// One device step of the major cycle (MajorCycle.cs:125-129 then :88-90):
+// ToUVW of the model image, DeGridW, residuals, grid the residuals, image.
+int cycle_step(p9idg_plan* p, int grid_mode) {
+    if (int st = do_image_to_grid(p)) return st;
+    if (int st = do_exec_degrid(p, P9IDG_DEGRID_WSTACK)) return st;
+    if (int st = do_residual(p)) return st;
+    const int32_t src = p->grid_src;
+    p->grid_src = P9IDG_SRC_RESIDUAL;
+    int st = do_exec_grid(p, grid_mode);
+    p->grid_src = src;
+    if (st) return st;
+    return do_grid_to_image(p, grid_mode);
+}
+
+void drop_graph(p9idg_plan* p) {
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
 }
 
 int upload_vis_any(p9idg_plan* p, const void* vis, int32_t type) {
@@ -419,6 +467,7 @@ int set_psf_vis(p9idg_plan* p, const uint8_t* flags) {
         if (int st = dev_alloc(&p->d_flags, n)) return st;
         P9_HIP(hipMemcpyAsync(p->d_flags, flags, n, hipMemcpyHostToDevice, p->stream));
         dflags = p->d_flags;
+        p->have_flags = true;
     }
     P9_HIP(launch_psf_vis(dflags, p->d_vis, n, p->stream));
     return P9IDG_OK;
@@ -556,6 +605,8 @@ int p9idg_destroy(p9idg_plan* p) {
     dev_free(p->d_phasor);
     dev_free(p->d_flags);
     dev_free(p->d_counter);
+    dev_free(p->d_resid);
+    drop_graph(p);
     dev_free(p->d_items_grid);
     dev_free(p->d_items_degrid);
     if (p->d_staging) (void)hipFree(p->d_staging);
@@ -632,10 +683,13 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
     }
     DeviceGuard dg(p->device);
     // drop the previous observation's buffers if the shape changed
+    drop_graph(p);
     if (p->have_obs && ((size_t)p->B * p->T * p->C != (size_t)B * T * C)) {
         dev_free(p->d_vis);
         dev_free(p->d_model_vis);
         dev_free(p->d_flags);
+        dev_free(p->d_resid);
+        p->have_flags = false;
     }
     if (p->have_obs && (size_t)p->B * p->T != (size_t)B * T) dev_free(p->d_uvw);
     if (p->have_obs && p->C != C) dev_free(p->d_wavenumbers);
@@ -876,6 +930,78 @@ int p9idg_exec_grid(p9idg_plan* p, int32_t mode) { return do_exec_grid(p, mode);
 int p9idg_exec_grid_to_image(p9idg_plan* p, int32_t mode) { return do_grid_to_image(p, mode); }
 int p9idg_exec_image_to_grid(p9idg_plan* p) { return do_image_to_grid(p); }
 int p9idg_exec_degrid(p9idg_plan* p, int32_t mode) { return do_exec_degrid(p, mode); }
+int p9idg_set_flags(p9idg_plan* p, const uint8_t* flags) {
+    if (int st = check_obs(p)) return st;
+    DeviceGuard dg(p->device);
+    if (!flags) {
+        p->have_flags = false;
+        return P9IDG_OK;
+    }
+    if (int st = dev_alloc(&p->d_flags, vis_elems(p))) return st;
+    P9_HIP(hipMemcpyAsync(p->d_flags, flags, vis_elems(p), hipMemcpyHostToDevice, p->stream));
+    p->have_flags = true;
+    return sync(p);
+}
+
+int p9idg_exec_residual(p9idg_plan* p) { return do_residual(p); }
+
+int p9idg_set_grid_source(p9idg_plan* p, int32_t src) {
+    if (int st = check_plan(p)) return st;
+    if (src != P9IDG_SRC_VIS && src != P9IDG_SRC_RESIDUAL) return fail(P9IDG_ERR_ARGUMENT, "unknown grid source");
+    p->grid_src = src;
+    return P9IDG_OK;
+}
+
+int p9idg_exec_cycle(p9idg_plan* p, int32_t grid_mode) {
+    if (int st = check_obs(p)) return st;
+    return cycle_step(p, grid_mode);
+}
+
+int p9idg_capture_cycle(p9idg_plan* p, int32_t grid_mode) {
+    if (int st = check_obs(p)) return st;
+    if (grid_mode == P9IDG_GRID_PLAIN && !p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+    DeviceGuard dg(p->device);
+    drop_graph(p);
+    // one eager step creates the FFT plans, buffers and occupancy figures
+    if (int st = cycle_step(p, grid_mode)) return st;
+    P9_HIP(hipStreamSynchronize(p->stream));
+    P9_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    p->capturing = true;
+    int st = cycle_step(p, grid_mode);
+    p->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    p->graph = g;
+    e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(p);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return P9IDG_OK;
+}
+
+int p9idg_launch_cycle(p9idg_plan* p) {
+    if (int st = check_plan(p)) return st;
+    if (!p->graph_exec) return fail(P9IDG_ERR_STATE, "no captured cycle (p9idg_capture_cycle)");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipGraphLaunch(p->graph_exec, p->stream));
+    return P9IDG_OK;
+}
+
+int p9idg_download_residual(p9idg_plan* p, void* out) {
+    if (int st = check_obs(p)) return st;
+    if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    if (!p->d_resid) return fail(P9IDG_ERR_STATE, "no residual visibilities");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipMemcpyAsync(out, p->d_resid, vis_elems(p) * sizeof(float2), hipMemcpyDeviceToHost, p->stream));
+    return sync(p);
+}
+
 int p9idg_sync(p9idg_plan* p) {
     if (int st = check_plan(p)) return st;
     DeviceGuard dg(p->device);

@@ -83,6 +83,17 @@ __global__ void scale_c64_kernel(float2* p, size_t n, double inv) {
     }
 }
 
+// Visibilities.Substract (Common.cs:305-318): vis - model, 0 where flagged.
+__global__ void residual_kernel(const float2* vis, const float2* model, const uint8_t* flags,
+                                float2* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const float2 v = vis[i], m = model[i];
+        out[i] = (flags && flags[i]) ? make_float2(0.f, 0.f) : make_float2(v.x - m.x, v.y - m.y);
+    }
+}
+
 inline unsigned blocks_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
@@ -140,6 +151,12 @@ hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid
                                 hipStream_t st) {
     hipLaunchKernelGGL(dev::image_to_grid_kernel, dim3(dev::blocks_for((size_t)G * G)),
                        dim3(256), 0, st, img, G, shift, grid);
+    return hipGetLastError();
+}
+hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t* flags,
+                           float2* out, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(dev::residual_kernel, dim3(dev::blocks_for(n)), dim3(256), 0, st, vis, model,
+                       flags, out, n);
     return hipGetLastError();
 }
 hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st) {

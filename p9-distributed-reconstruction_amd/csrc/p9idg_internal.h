@@ -99,6 +99,8 @@ hipError_t launch_grid_to_image_f64(const float2* grid, int G, double inv_n, int
                                     double* img, hipStream_t st);
 hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid,
                                 hipStream_t st);
+hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t* flags,
+                           float2* out, size_t n, hipStream_t st);
 hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st);
 
 }  // namespace p9idg

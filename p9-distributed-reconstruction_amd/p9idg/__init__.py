@@ -21,4 +21,6 @@ from ._lib import (  # noqa: F401
     GRID_PLAIN,
     GRID_WSTACK,
     GRID_WSUM,
+    SRC_RESIDUAL,
+    SRC_VIS,
 )

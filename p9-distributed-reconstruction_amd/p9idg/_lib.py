@@ -37,6 +37,8 @@ GRID_WSTACK = 1
 GRID_WSUM = 2
 DEGRID_PLAIN = 0
 DEGRID_WSTACK = 1
+SRC_VIS = 0
+SRC_RESIDUAL = 1
 
 SUBGRID_DTYPE = np.dtype(
     [
@@ -119,6 +121,13 @@ IDG_SIGNATURES = {
     "p9idg_exec_image_to_grid": [_vp],
     "p9idg_exec_degrid": [_vp, _i32],
     "p9idg_sync": [_vp],
+    "p9idg_set_flags": [_vp, _vp],
+    "p9idg_exec_residual": [_vp],
+    "p9idg_set_grid_source": [_vp, _i32],
+    "p9idg_exec_cycle": [_vp, _i32],
+    "p9idg_capture_cycle": [_vp, _i32],
+    "p9idg_launch_cycle": [_vp],
+    "p9idg_download_residual": [_vp, _vp],
     "p9idg_device_buffers": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "p9idg_download_grid": [_vp, _i32, _vp],
     "p9idg_upload_model_grid": [_vp, _vp],

@@ -285,6 +285,30 @@ class Plan:
     def sync(self):
         check(_lib.idg().p9idg_sync(self._h))
 
+    def set_flags(self, flags=None):
+        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        check(_lib.idg().p9idg_set_flags(self._h, ptr(fl)))
+
+    def exec_residual(self):
+        check(_lib.idg().p9idg_exec_residual(self._h))
+
+    def set_grid_source(self, src):
+        check(_lib.idg().p9idg_set_grid_source(self._h, int(src)))
+
+    def exec_cycle(self, mode=_lib.GRID_WSUM):
+        check(_lib.idg().p9idg_exec_cycle(self._h, int(mode)))
+
+    def capture_cycle(self, mode=_lib.GRID_WSUM):
+        check(_lib.idg().p9idg_capture_cycle(self._h, int(mode)))
+
+    def launch_cycle(self):
+        check(_lib.idg().p9idg_launch_cycle(self._h))
+
+    def download_residual(self):
+        out = np.zeros(self.shape, dtype=np.complex64)
+        check(_lib.idg().p9idg_download_residual(self._h, ptr(out)))
+        return out
+
     def download_grid(self, mode=_lib.GRID_PLAIN):
         G, L = self.c.GridSize, self.c.WLayerCount
         shape = (L, G, G) if mode == _lib.GRID_WSTACK else (G, G)

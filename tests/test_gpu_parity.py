@@ -255,3 +255,40 @@ def test_wsum_mode_matches_wstack_image(large):
         plan.exec_grid_to_image(p9idg.GRID_WSUM)
         b = plan.download_image()
     assert rel_err(b, a) <= 1e-5
+
+
+@pytest.mark.parametrize("kernel", ["fp32", "mfma_bf16"])
+def test_major_cycle_step_eager_graph_and_oracle(small, O, kernel):
+    """One device step of the major cycle (ToUVW -> DeGridW -> residuals ->
+    GridW -> image), run eagerly and as a replayed hipGraph, against the
+    oracle's MajorCycle.cs:125-129 + :88-90 sequence."""
+    ob, part, oc = small
+    G = ob.c.GridSize
+    rng = np.random.default_rng(4)
+    model = np.zeros((G, G), np.float32)
+    model[G // 2 - 12, G // 2 + 20] = 0.7  # near the brighter source
+    model[rng.integers(0, G, 5), rng.integers(0, G, 5)] = 0.05
+    flags = rng.random(ob.vis.shape) < 0.05
+    model_vis = O.image_to_vis(oc, part.subgrids, model, ob.uvw, ob.freq)
+    resid = O.substract(ob.vis, model_vis, flags)
+    ref = O.gridw_to_image(oc, part.subgrids, resid, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.set_flags(flags)
+        plan.upload_image(model)
+        plan.exec_cycle(p9idg.GRID_WSTACK)
+        eager = plan.download_image()
+        assert rel_err(eager, ref) <= TOL
+        assert rel_err(plan.download_residual(), resid) <= TOL
+        plan.upload_image(model)
+        plan.capture_cycle(p9idg.GRID_WSTACK)
+        for rep in range(3):
+            plan.upload_image(model)  # eager work between replays
+            plan.exec_grid(p9idg.GRID_PLAIN)
+            plan.sync()
+            plan.upload_image(model)
+            plan.launch_cycle()
+            plan.sync()
+            img = plan.download_image()
+            assert rel_err(img, eager) <= 1e-6, rep
