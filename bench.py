@@ -194,6 +194,7 @@ def main():
     ap.add_argument("--kernel", default="fp32", choices=sorted(PEAK_TFLOPS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--full-accuracy", action="store_true",
                     help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
@@ -225,6 +226,10 @@ def main():
     plan.sync()
     n_vis = plan.gridded_visibilities
     log("gridded visibilities: %d of %d" % (n_vis, ob.vis.size))
+    tuning = plan.get_launch()
+    if not args.no_autotune:
+        tuning = plan.autotune(reps=3)
+        log("autotuned launch shape: %s" % tuning)
 
     def step():
         plan.exec_grid(p9idg.GRID_WSTACK)
@@ -295,6 +300,7 @@ def main():
             "gridded_visibilities": int(n_vis),
             "total_visibilities": int(ob.vis.size),
             "subgrids": int(part.total),
+            "launch": tuning,
         },
         "stages": {
             "gridder_ms": avg_g,

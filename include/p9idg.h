@@ -147,6 +147,12 @@ int p9idg_set_launch(p9idg_plan* plan, int32_t queue, int32_t grid_chunk, int32_
                      int32_t workgroups);
 int p9idg_get_launch(const p9idg_plan* plan, int32_t* queue, int32_t* grid_chunk,
                      int32_t* degrid_chunk, int32_t* workgroups);
+/* Chooses the launch shape (queue or not, visibilities per work item for
+ * each direction, persistent workgroups) by timing candidates on the plan's
+ * observation with HIP events (median of reps runs each); keeps the fastest
+ * and returns its gridder / degridder times.  Overwrites the grid and model
+ * visibility buffers. */
+int p9idg_autotune(p9idg_plan* plan, int32_t reps, float* grid_ms, float* degrid_ms);
 /* Uploads uvw, frequencies and the partition to HBM (once per observation). */
 int p9idg_set_observation(p9idg_plan* plan, const double* uvw, int32_t nr_baselines,
                           int32_t nr_timesteps, const double* freq, int32_t nr_channels,

@@ -993,6 +993,78 @@ int p9idg_launch_cycle(p9idg_plan* p) {
     return P9IDG_OK;
 }
 
+// Times one gridder (dir 0) or degridder (dir 1) launch with the plan's
+// current launch shape; median of `reps` runs.
+static int time_direction(p9idg_plan* p, int dir, int reps, float* ms_out) {
+    std::vector<float> t;
+    for (int r = 0; r < reps; r++) {
+        int st = dir == 0 ? do_exec_grid(p, P9IDG_GRID_WSUM) : do_exec_degrid(p, P9IDG_DEGRID_WSTACK);
+        if (st) return st;
+        P9_HIP(hipStreamSynchronize(p->stream));
+        float ms = 0.f;
+        P9_HIP(hipEventElapsedTime(&ms, p->ev[dir == 0 ? 1 : 3], p->ev[dir == 0 ? 2 : 4]));
+        t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    *ms_out = t[t.size() / 2];
+    return P9IDG_OK;
+}
+
+int p9idg_autotune(p9idg_plan* p, int32_t reps, float* grid_ms, float* degrid_ms) {
+    if (int st = check_obs(p)) return st;
+    if (reps < 1) reps = 1;
+    DeviceGuard dg(p->device);
+    const bool collect = p->collect;
+    p->collect = false;
+    int cus = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) cus = 1;
+    // candidate shapes: one workgroup per subgrid, or the queue with
+    // items of `chunk` visibilities on `wg` persistent workgroups
+    struct Cand {
+        int queue, chunk, wg;
+    };
+    std::vector<Cand> cands = {{0, 0, 0}};
+    for (int chunk : {512, 1024, 2048, 4096, 0})
+        for (int wg : {0, cus, 2 * cus, 4 * cus}) cands.push_back({1, chunk, wg});
+    float best[2] = {1e30f, 1e30f};
+    Cand pick[2] = {{p->use_queue, p->grid_chunk, p->workgroups}, {p->use_queue, p->degrid_chunk, p->workgroups}};
+    const int32_t q0 = p->use_queue, g0 = p->grid_chunk, d0 = p->degrid_chunk, w0 = p->workgroups;
+    for (const Cand& c : cands) {
+        p->use_queue = c.queue;
+        p->grid_chunk = c.chunk;
+        p->degrid_chunk = c.chunk;
+        p->workgroups = c.wg;
+        if (int st = upload_items(p)) return st;
+        for (int dir = 0; dir < 2; dir++) {
+            float ms = 0.f;
+            if (int st = time_direction(p, dir, reps, &ms)) {
+                p->collect = collect;
+                return st;
+            }
+            if (ms < best[dir]) {
+                best[dir] = ms;
+                pick[dir] = c;
+            }
+        }
+    }
+    (void)q0;
+    (void)g0;
+    (void)d0;
+    (void)w0;
+    // the queue flag and workgroup count are shared by both directions: take
+    // them from the direction with the larger time, chunks from each
+    const int dom = best[0] >= best[1] ? 0 : 1;
+    p->use_queue = pick[dom].queue;
+    p->workgroups = pick[dom].wg;
+    p->grid_chunk = pick[0].chunk;
+    p->degrid_chunk = pick[1].chunk;
+    p->collect = collect;
+    if (int st = upload_items(p)) return st;
+    if (grid_ms) *grid_ms = best[0];
+    if (degrid_ms) *degrid_ms = best[1];
+    return P9IDG_OK;
+}
+
 int p9idg_download_residual(p9idg_plan* p, void* out) {
     if (int st = check_obs(p)) return st;
     if (!out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");

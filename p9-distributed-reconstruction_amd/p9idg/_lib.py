@@ -97,6 +97,7 @@ IDG_SIGNATURES = {
     "p9idg_get_kernel": [_vp, ctypes.POINTER(_i32)],
     "p9idg_set_observation": [_vp, _vp, _i32, _i32, _vp, _i32, _vp, _i64],
     "p9idg_set_launch": [_vp, _i32, _i32, _i32, _i32],
+    "p9idg_autotune": [_vp, _i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "p9idg_get_launch": [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     "p9idg_grid": [_vp, _vp, _i32, _vp],
     "p9idg_degrid": [_vp, _vp, _vp],

@@ -187,6 +187,14 @@ class Plan:
     def set_launch(self, queue=True, grid_chunk=2048, degrid_chunk=2048, workgroups=0):
         check(_lib.idg().p9idg_set_launch(self._h, 1 if queue else 0, int(grid_chunk), int(degrid_chunk), int(workgroups)))
 
+    def autotune(self, reps=3):
+        """Times candidate launch shapes on this observation and keeps the fastest."""
+        g, d = ctypes.c_float(), ctypes.c_float()
+        check(_lib.idg().p9idg_autotune(self._h, int(reps), ctypes.byref(g), ctypes.byref(d)))
+        out = self.get_launch()
+        out.update(grid_ms=g.value, degrid_ms=d.value)
+        return out
+
     def get_launch(self):
         v = [ctypes.c_int32() for _ in range(4)]
         check(_lib.idg().p9idg_get_launch(self._h, *[ctypes.byref(x) for x in v]))
