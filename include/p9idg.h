@@ -251,6 +251,26 @@ int p9idg_timing_totals(p9idg_plan* plan, double* grid_ms, int64_t* grid_launche
 /* Number of visibilities covered by the partition (sum of count * C). */
 int p9idg_gridded_visibilities(p9idg_plan* plan, int64_t* n);
 
+/* ---- Batches: several observations in one persistent launch -------------- */
+/* The plans keep their observations and buffers; a batch groups them by
+ * (subgrid size, kernel) and runs each group's work items -- the K-split
+ * subgrids of all its observations, longest first -- as one persistent
+ * work-queued launch per direction, on the batch's own stream. */
+typedef struct p9idg_batch p9idg_batch;
+int p9idg_batch_create(p9idg_plan* const* plans, int32_t nr_plans, p9idg_batch** batch);
+int p9idg_batch_destroy(p9idg_batch* batch);
+/* Builds the merged work queues for the given modes (after every plan's
+ * observation and kernel are set). */
+int p9idg_batch_prepare(p9idg_batch* batch, int32_t grid_mode, int32_t degrid_mode);
+/* what: 1 = grid (zero + gridder), 2 = degrid (zero + degridder), 3 = both */
+int p9idg_batch_exec(p9idg_batch* batch, int32_t what);
+/* Records p9idg_batch_exec(what) into a hipGraph; launch replays it. */
+int p9idg_batch_capture(p9idg_batch* batch, int32_t what);
+int p9idg_batch_launch(p9idg_batch* batch);
+int p9idg_batch_sync(p9idg_batch* batch);
+/* kernel-only milliseconds of the last eager p9idg_batch_exec(batch, 3) */
+int p9idg_batch_timings(p9idg_batch* batch, float* grid_ms, float* degrid_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1200,6 +1200,254 @@ int p9idg_timing_totals(p9idg_plan* p, double* grid_ms, int64_t* grid_launches,
     return P9IDG_OK;
 }
 
+}  // extern "C"
+
+// ---- batches: several observations per persistent launch --------------------
+struct p9idg_batch {
+    std::vector<p9idg_plan*> plans;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    struct Group {
+        int S = 0, kernel = 0, workgroups = 0;
+        std::vector<int> members;  // plan indices; item.inst indexes this list
+        KernelArgs* d_args_grid = nullptr;
+        KernelArgs* d_args_degrid = nullptr;
+        WorkItem* d_items_grid = nullptr;
+        WorkItem* d_items_degrid = nullptr;
+        int32_t n_grid = 0, n_degrid = 0;
+        int32_t* d_counter = nullptr;
+    };
+    std::vector<Group> groups;
+    int grid_mode = P9IDG_GRID_WSTACK, degrid_mode = P9IDG_DEGRID_WSTACK;
+    bool prepared = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t ev[5] = {};
+    bool timed = false;
+};
+
+namespace {
+
+void batch_free_groups(p9idg_batch* b) {
+    for (auto& g : b->groups) {
+        dev_free(g.d_args_grid);
+        dev_free(g.d_args_degrid);
+        dev_free(g.d_items_grid);
+        dev_free(g.d_items_degrid);
+        dev_free(g.d_counter);
+    }
+    b->groups.clear();
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->prepared = false;
+}
+
+// Zero fills and one persistent launch per (subgrid size, kernel) group.
+int batch_run(p9idg_batch* b, int what, bool capturing) {
+    if (!b->prepared) return fail(P9IDG_ERR_STATE, "batch not prepared (p9idg_batch_prepare)");
+    if (!capturing) P9_HIP(hipEventRecord(b->ev[0], b->stream));
+    if (what & 1) {
+        for (p9idg_plan* p : b->plans) {
+            float2* target = b->grid_mode == P9IDG_GRID_WSTACK ? p->d_layers : p->d_grid;
+            size_t n = grid_elems(p) * (b->grid_mode == P9IDG_GRID_WSTACK ? p->c.wlayer_count : 1);
+            P9_HIP(launch_fill_zero(target, n, b->stream));
+        }
+        if (!capturing) P9_HIP(hipEventRecord(b->ev[1], b->stream));
+        for (auto& g : b->groups) {
+            BatchArgs ba{g.d_args_grid, g.d_items_grid, g.n_grid, g.d_counter};
+            hipError_t e = launch_grid_batch(ba, g.S, g.kernel, g.workgroups, b->stream);
+            if (e != hipSuccess) return hip_fail(e, "batched gridder launch");
+        }
+    }
+    if (!capturing) P9_HIP(hipEventRecord(b->ev[2], b->stream));
+    if (what & 2) {
+        for (p9idg_plan* p : b->plans) P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), b->stream));
+        if (!capturing) P9_HIP(hipEventRecord(b->ev[3], b->stream));
+        for (auto& g : b->groups) {
+            BatchArgs ba{g.d_args_degrid, g.d_items_degrid, g.n_degrid, g.d_counter};
+            hipError_t e = launch_degrid_batch(ba, g.S, g.kernel, g.workgroups, b->stream);
+            if (e != hipSuccess) return hip_fail(e, "batched degridder launch");
+        }
+    }
+    if (!capturing) {
+        P9_HIP(hipEventRecord(b->ev[4], b->stream));
+        b->timed = (what & 1) && (what & 2);
+    }
+    return P9IDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int p9idg_batch_create(p9idg_plan* const* plans, int32_t n, p9idg_batch** out) {
+    if (!plans || !out || n <= 0) return fail(P9IDG_ERR_ARGUMENT, "invalid batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!plans[i]) return fail(P9IDG_ERR_ARGUMENT, "NULL plan in batch");
+        if (!plans[i]->have_obs) return fail(P9IDG_ERR_STATE, "plan without observation in batch");
+        if (plans[i]->device != plans[0]->device) return fail(P9IDG_ERR_ARGUMENT, "plans on different devices");
+    }
+    auto* b = new p9idg_batch();
+    b->plans.assign(plans, plans + n);
+    b->device = plans[0]->device;
+    DeviceGuard dg(b->device);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    for (auto& ev : b->ev) (void)hipEventCreate(&ev);
+    *out = b;
+    return P9IDG_OK;
+}
+
+int p9idg_batch_destroy(p9idg_batch* b) {
+    if (!b) return P9IDG_OK;
+    DeviceGuard dg(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    batch_free_groups(b);
+    for (auto& ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return P9IDG_OK;
+}
+
+int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) {
+    if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
+    if (grid_mode < P9IDG_GRID_PLAIN || grid_mode > P9IDG_GRID_WSUM || degrid_mode < 0 || degrid_mode > 1)
+        return fail(P9IDG_ERR_ARGUMENT, "unknown mode");
+    DeviceGuard dg(b->device);
+    batch_free_groups(b);
+    b->grid_mode = grid_mode;
+    b->degrid_mode = degrid_mode;
+    for (int i = 0; i < (int)b->plans.size(); i++) {
+        p9idg_plan* p = b->plans[i];
+        if (grid_mode == P9IDG_GRID_PLAIN && !p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+        if (grid_mode == P9IDG_GRID_WSTACK)
+            if (int st = ensure_layers(p)) return st;
+        P9_HIP(hipStreamSynchronize(p->stream));
+        bool found = false;
+        for (auto& g : b->groups)
+            if (g.S == p->c.subgrid_size && g.kernel == p->kernel) {
+                g.members.push_back(i);
+                found = true;
+            }
+        if (!found) {
+            p9idg_batch::Group g;
+            g.S = p->c.subgrid_size;
+            g.kernel = p->kernel;
+            g.workgroups = p->workgroups;
+            g.members.push_back(i);
+            b->groups.push_back(g);
+        }
+    }
+    for (auto& g : b->groups) {
+        std::vector<KernelArgs> ag, ad;
+        std::vector<WorkItem> ig, id;
+        for (int m = 0; m < (int)g.members.size(); m++) {
+            p9idg_plan* p = b->plans[g.members[m]];
+            KernelArgs a = make_args(p, grid_mode);
+            a.vis = p->d_vis;
+            a.grid = grid_mode == P9IDG_GRID_WSTACK ? p->d_layers : p->d_grid;
+            ag.push_back(a);
+            KernelArgs d = make_args(p, degrid_mode);
+            ad.push_back(d);
+            for (WorkItem w : make_items(p->meta_host, p->order_host, p->C, p->grid_chunk)) {
+                w.inst = m;
+                ig.push_back(w);
+            }
+            for (WorkItem w : make_items(p->meta_host, p->order_host, p->C, p->degrid_chunk)) {
+                w.inst = m;
+                id.push_back(w);
+            }
+        }
+        auto longest = [](const WorkItem& x, const WorkItem& y) { return x.k1 - x.k0 > y.k1 - y.k0; };
+        std::stable_sort(ig.begin(), ig.end(), longest);
+        std::stable_sort(id.begin(), id.end(), longest);
+        int st;
+        if ((st = dev_alloc(&g.d_args_grid, ag.size())) || (st = dev_alloc(&g.d_args_degrid, ad.size())) ||
+            (st = dev_alloc(&g.d_items_grid, std::max<size_t>(ig.size(), 1))) ||
+            (st = dev_alloc(&g.d_items_degrid, std::max<size_t>(id.size(), 1))) || (st = dev_alloc(&g.d_counter, (size_t)1)))
+            return st;
+        P9_HIP(hipMemcpy(g.d_args_grid, ag.data(), ag.size() * sizeof(KernelArgs), hipMemcpyHostToDevice));
+        P9_HIP(hipMemcpy(g.d_args_degrid, ad.data(), ad.size() * sizeof(KernelArgs), hipMemcpyHostToDevice));
+        if (!ig.empty()) P9_HIP(hipMemcpy(g.d_items_grid, ig.data(), ig.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        if (!id.empty()) P9_HIP(hipMemcpy(g.d_items_degrid, id.data(), id.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        g.n_grid = (int32_t)ig.size();
+        g.n_degrid = (int32_t)id.size();
+    }
+    b->prepared = true;
+    return P9IDG_OK;
+}
+
+/* what: 1 = grid, 2 = degrid, 3 = both */
+int p9idg_batch_exec(p9idg_batch* b, int32_t what) {
+    if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
+    DeviceGuard dg(b->device);
+    return batch_run(b, what, false);
+}
+
+int p9idg_batch_capture(p9idg_batch* b, int32_t what) {
+    if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
+    DeviceGuard dg(b->device);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    if (int st = batch_run(b, what, false)) return st;  // warm: occupancy figures, first launches
+    P9_HIP(hipStreamSynchronize(b->stream));
+    P9_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int st = batch_run(b, what, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return P9IDG_OK;
+}
+
+int p9idg_batch_launch(p9idg_batch* b) {
+    if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
+    if (!b->graph_exec) return fail(P9IDG_ERR_STATE, "no captured batch graph");
+    DeviceGuard dg(b->device);
+    P9_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+    return P9IDG_OK;
+}
+
+int p9idg_batch_sync(p9idg_batch* b) {
+    if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
+    DeviceGuard dg(b->device);
+    P9_HIP(hipStreamSynchronize(b->stream));
+    return P9IDG_OK;
+}
+
+/* kernel-only times of the last eager p9idg_batch_exec(b, 3) */
+int p9idg_batch_timings(p9idg_batch* b, float* grid_ms, float* degrid_ms) {
+    if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
+    if (!b->timed) return fail(P9IDG_ERR_STATE, "no timed batch run");
+    DeviceGuard dg(b->device);
+    P9_HIP(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    if (grid_ms) {
+        P9_HIP(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]));
+        *grid_ms = ms;
+    }
+    if (degrid_ms) {
+        P9_HIP(hipEventElapsedTime(&ms, b->ev[3], b->ev[4]));
+        *degrid_ms = ms;
+    }
+    return P9IDG_OK;
+}
+
 int p9idg_gridded_visibilities(p9idg_plan* p, int64_t* n) {
     if (int st = check_obs(p)) return st;
     if (!n) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");

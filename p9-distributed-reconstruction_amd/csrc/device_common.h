@@ -218,6 +218,22 @@ __device__ __forceinline__ void run_queue(const KernelArgs& a, Body body) {
     }
 }
 
+// Batched queue: like run_queue, with each item's observation in args[inst].
+template <class Body>
+__device__ __forceinline__ void run_batch_queue(const BatchArgs& b, Body body) {
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(b.counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= b.nitems) return;
+        const WorkItem w = b.items[it];
+        const KernelArgs a = b.args[w.inst];
+        body(a, w.sidx, w.k0, w.k1);
+    }
+}
+
 // Index of grid row/column r after FFT.Shift (quadrant swap) for even G.
 __device__ __forceinline__ int shifted(int r, int G) {
     const int h = G / 2;

@@ -147,6 +147,15 @@ __global__ __launch_bounds__(kThreads) void degrid_fp32_queue_kernel(KernelArgs 
     run_queue(a, [&](int s, int k0, int k1) { degrid_fp32_body<S>(a, s, k0, k1); });
 }
 
+template <int S>
+__global__ __launch_bounds__(kThreads) void grid_fp32_batch_kernel(BatchArgs b) {
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_fp32_body<S>(a, s, k0, k1); });
+}
+template <int S>
+__global__ __launch_bounds__(kThreads) void degrid_fp32_batch_kernel(BatchArgs b) {
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { degrid_fp32_body<S>(a, s, k0, k1); });
+}
+
 __global__ void reset_counter_kernel(int32_t* counter) { *counter = 0; }
 
 }  // namespace dev
@@ -191,6 +200,25 @@ hipError_t launch_degrid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, 
 #define P9_D(SV) P9_LAUNCH_FP32(dev::degrid_fp32_kernel, dev::degrid_fp32_queue_kernel, SV)
     P9_SWITCH_S(S, P9_D)
 #undef P9_D
+    return hipGetLastError();
+}
+
+hipError_t launch_fp32_batch(bool grid, const BatchArgs& b, int S, int workgroups, hipStream_t st) {
+#define P9_B(SV)                                                                                   \
+    do {                                                                                           \
+        const void* k = grid ? (const void*)dev::grid_fp32_batch_kernel<SV>                        \
+                             : (const void*)dev::degrid_fp32_batch_kernel<SV>;                     \
+        int wg = workgroups > 0 ? workgroups : persistent_workgroups(k, dev::kThreads);            \
+        wg = wg < b.nitems ? wg : b.nitems;                                                        \
+        if (wg <= 0) return hipSuccess;                                                            \
+        hipLaunchKernelGGL(dev::reset_counter_kernel, dim3(1), dim3(1), 0, st, b.counter);         \
+        if (grid)                                                                                  \
+            hipLaunchKernelGGL(dev::grid_fp32_batch_kernel<SV>, dim3(wg), dim3(dev::kThreads), 0, st, b); \
+        else                                                                                       \
+            hipLaunchKernelGGL(dev::degrid_fp32_batch_kernel<SV>, dim3(wg), dim3(dev::kThreads), 0, st, b); \
+    } while (0)
+    P9_SWITCH_S(S, P9_B)
+#undef P9_B
     return hipGetLastError();
 }
 

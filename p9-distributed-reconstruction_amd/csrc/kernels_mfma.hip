@@ -531,6 +531,15 @@ __global__ __launch_bounds__(kThreads) void degrid_mfma_queue_kernel(KernelArgs 
     run_queue(a, [&](int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
 }
 
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void grid_mfma_batch_kernel(BatchArgs b) {
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
+}
+template <int S, class P>
+__global__ __launch_bounds__(kThreads) void degrid_mfma_batch_kernel(BatchArgs b) {
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
+}
+
 }  // namespace dev
 
 #define P9_LAUNCH_MFMA(KERNEL, QKERNEL, SV, PREC)                                                  \
@@ -566,6 +575,48 @@ hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const Lau
     const bool f16 = precision == P9IDG_KERNEL_MFMA_FP16;
     P9_MFMA_SWITCH(dev::grid_mfma_kernel, dev::grid_mfma_queue_kernel)
     return hipGetLastError();
+}
+
+namespace {
+template <class K>
+hipError_t launch_batch_kernel(K kernel, const BatchArgs& b, int workgroups, hipStream_t st) {
+    int wg = workgroups > 0 ? workgroups : persistent_workgroups((const void*)kernel, dev::kThreads);
+    wg = wg < b.nitems ? wg : b.nitems;
+    if (wg <= 0) return hipSuccess;
+    hipError_t e = launch_reset_counter(b.counter, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kernel, dim3(wg), dim3(dev::kThreads), 0, st, b);
+    return hipGetLastError();
+}
+
+template <int S>
+hipError_t batch_dispatch(bool grid, const BatchArgs& b, int kernel, int wg, hipStream_t st) {
+    if (kernel == P9IDG_KERNEL_FP32) return launch_fp32_batch(grid, b, S, wg, st);
+    if (kernel == P9IDG_KERNEL_MFMA_FP16)
+        return grid ? launch_batch_kernel(dev::grid_mfma_batch_kernel<S, dev::PrecF16>, b, wg, st)
+                    : launch_batch_kernel(dev::degrid_mfma_batch_kernel<S, dev::PrecF16>, b, wg, st);
+    return grid ? launch_batch_kernel(dev::grid_mfma_batch_kernel<S, dev::PrecBF16>, b, wg, st)
+                : launch_batch_kernel(dev::degrid_mfma_batch_kernel<S, dev::PrecBF16>, b, wg, st);
+}
+
+hipError_t batch_switch(bool grid, const BatchArgs& b, int S, int kernel, int wg, hipStream_t st) {
+    switch (S) {
+        case 8: return batch_dispatch<8>(grid, b, kernel, wg, st);
+        case 16: return batch_dispatch<16>(grid, b, kernel, wg, st);
+        case 24: return batch_dispatch<24>(grid, b, kernel, wg, st);
+        case 32: return batch_dispatch<32>(grid, b, kernel, wg, st);
+        case 48: return batch_dispatch<48>(grid, b, kernel, wg, st);
+        case 64: return batch_dispatch<64>(grid, b, kernel, wg, st);
+        default: return hipErrorNotSupported;
+    }
+}
+}  // namespace
+
+hipError_t launch_grid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st) {
+    return batch_switch(true, b, S, kernel, workgroups, st);
+}
+hipError_t launch_degrid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st) {
+    return batch_switch(false, b, S, kernel, workgroups, st);
 }
 
 hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,

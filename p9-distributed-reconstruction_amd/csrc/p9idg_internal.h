@@ -80,6 +80,14 @@ struct KernelArgs {
     int32_t* counter;      // next item; zeroed by the launcher before each launch
 };
 
+// Several observations in one persistent launch: item.inst indexes args.
+struct BatchArgs {
+    const KernelArgs* args;
+    const WorkItem* items;
+    int32_t nitems;
+    int32_t* counter;
+};
+
 // Launchers (kernels_*.hip); S is the subgrid size.
 hipError_t launch_grid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_degrid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st);
@@ -88,6 +96,10 @@ hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const Lau
 hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
                               hipStream_t st);
 hipError_t launch_reset_counter(int32_t* counter, hipStream_t st);
+hipError_t launch_fp32_batch(bool grid, const BatchArgs& b, int S, int workgroups, hipStream_t st);
+// kernel: p9idg_kernel; workgroups 0 = CUs x occupancy
+hipError_t launch_grid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st);
+hipError_t launch_degrid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st);
 // Persistent workgroups for a kernel: CUs x resident workgroups per CU.
 int persistent_workgroups(const void* kernel, int threads);
 hipError_t launch_fill_zero(float2* p, size_t n, hipStream_t st);

@@ -5,6 +5,7 @@ backed by the HIP library libp9idg.so; see p9idg.idg.
 """
 from .idg import (  # noqa: F401
     FFT,
+    Batch,
     IDG,
     GriddingConstants,
     P9IDGError,

@@ -140,6 +140,14 @@ IDG_SIGNATURES = {
     "p9idg_timing_reset": [_vp, _i32],
     "p9idg_timing_totals": [_vp, ctypes.POINTER(_f64), _pi64, ctypes.POINTER(_f64), _pi64],
     "p9idg_gridded_visibilities": [_vp, _pi64],
+    "p9idg_batch_create": [_vp, _i32, ctypes.POINTER(_vp)],
+    "p9idg_batch_destroy": [_vp],
+    "p9idg_batch_prepare": [_vp, _i32, _i32],
+    "p9idg_batch_exec": [_vp, _i32],
+    "p9idg_batch_capture": [_vp, _i32],
+    "p9idg_batch_launch": [_vp],
+    "p9idg_batch_sync": [_vp],
+    "p9idg_batch_timings": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
 }
 _RESTYPES = {"p9idg_last_error": ctypes.c_char_p}
 

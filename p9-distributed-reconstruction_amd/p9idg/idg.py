@@ -33,6 +33,7 @@ __all__ = [
     "FFT",
     "Visibilities",
     "Plan",
+    "Batch",
     "P9IDGError",
     "SUBGRID_DTYPE",
 ]
@@ -369,6 +370,52 @@ class Plan:
         n = ctypes.c_int64()
         check(_lib.idg().p9idg_gridded_visibilities(self._h, ctypes.byref(n)))
         return n.value
+
+
+class Batch:
+    """Several plans' gridding / degridding as one persistent work-queued
+    launch per (subgrid size, kernel) group, optionally replayed as a hipGraph."""
+
+    def __init__(self, plans, grid_mode=_lib.GRID_WSTACK, degrid_mode=_lib.DEGRID_WSTACK):
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        self._h = ctypes.c_void_p()
+        check(_lib.idg().p9idg_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(self.plans), ctypes.byref(self._h)))
+        check(_lib.idg().p9idg_batch_prepare(self._h, int(grid_mode), int(degrid_mode)))
+
+    def exec(self, what=3):
+        check(_lib.idg().p9idg_batch_exec(self._h, int(what)))
+
+    def capture(self, what=3):
+        check(_lib.idg().p9idg_batch_capture(self._h, int(what)))
+
+    def launch(self):
+        check(_lib.idg().p9idg_batch_launch(self._h))
+
+    def sync(self):
+        check(_lib.idg().p9idg_batch_sync(self._h))
+
+    def timings(self):
+        g, d = ctypes.c_float(), ctypes.c_float()
+        check(_lib.idg().p9idg_batch_timings(self._h, ctypes.byref(g), ctypes.byref(d)))
+        return {"grid_ms": g.value, "degrid_ms": d.value}
+
+    def close(self):
+        if self._h:
+            _lib.idg().p9idg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class _PlanCache:

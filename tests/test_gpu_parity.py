@@ -292,3 +292,48 @@ def test_major_cycle_step_eager_graph_and_oracle(small, O, kernel):
             plan.sync()
             img = plan.download_image()
             assert rel_err(img, eager) <= 1e-6, rep
+
+
+def test_batched_uneven_instances_match_oracle(O):
+    """Observations with subgrid sizes 24, 32 and 48 in one batch: one
+    persistent launch per size class, eager and replayed from a hipGraph."""
+    obsv = [
+        obs.custom(G=256, S=24, K=8, L=4, n_ant=9, T=30, C=5, seed=61),
+        obs.custom(G=512, S=32, K=8, L=4, n_ant=12, T=60, C=8, seed=62),
+        obs.custom(G=256, S=48, K=8, L=2, n_ant=7, T=20, C=4, seed=63),
+        obs.custom(G=384, S=32, K=8, L=4, n_ant=10, T=45, C=6, seed=64),
+    ]
+    plans, refs = [], []
+    try:
+        for ob in obsv:
+            part = ob.partition()
+            oc = oracle_constants(ob.c)
+            ref_l = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
+            model = ref_l.sum(axis=0)
+            ref_v = O.degridw(oc, part.subgrids, model, ob.uvw, ob.freq)
+            p = p9idg.Plan(ob.c, kernel="mfma_bf16")
+            p.set_launch(queue=True, grid_chunk=128, degrid_chunk=160)
+            p.set_observation(ob.uvw, ob.freq, part)
+            p.upload_vis(ob.vis)
+            p.upload_model_grid(model.astype(np.complex64))
+            plans.append(p)
+            refs.append((ref_l, ref_v))
+        with p9idg.Batch(plans) as batch:
+            batch.exec(3)
+            batch.sync()
+            for p, (ref_l, ref_v) in zip(plans, refs):
+                assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
+                assert rel_err(p.download_model_vis(), ref_v) <= TOL
+            batch.capture(3)
+            for rep in range(2):
+                for p in plans:  # eager work between replays
+                    p.exec_grid(p9idg.GRID_PLAIN)
+                    p.sync()
+                batch.launch()
+                batch.sync()
+                for p, (ref_l, ref_v) in zip(plans, refs):
+                    assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL, rep
+                    assert rel_err(p.download_model_vis(), ref_v) <= TOL, rep
+    finally:
+        for p in plans:
+            p.close()
