@@ -186,6 +186,85 @@ def full_accuracy(ob, part, model_grid_c64, kernel):
     return out
 
 
+def run_batch(args):
+    """The 8 uneven seeded observations (p9idg/observation.py batch_instance)
+    gridded and degridded as one persistent launch per subgrid-size class,
+    replayed from a hipGraph; per-instance error on a subgrid sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import p9idg
+    from p9idg import observation
+
+    O.set_threads(host_threads())
+    plans, obs_list, parts = [], [], []
+    t = time.perf_counter()
+    for i in range(8):
+        ob = observation.batch_instance(i, threads=host_threads())
+        part = ob.partition()
+        p = p9idg.Plan(ob.c, kernel=args.kernel)
+        p.set_observation(ob.uvw, ob.freq, part)
+        p.upload_vis(ob.vis)
+        p.upload_image(model_image(ob))
+        p.exec_image_to_grid()
+        if not args.no_autotune:
+            p.autotune(reps=2)
+        plans.append(p)
+        obs_list.append(ob)
+        parts.append(part)
+    log("batch: 8 instances ready in %.1f s" % (time.perf_counter() - t))
+    n_vis = sum(p.gridded_visibilities for p in plans)
+    batch = p9idg.Batch(plans)
+    for _ in range(args.warmup):
+        batch.exec(3)
+    batch.exec(3)
+    batch.sync()
+    kt = batch.timings()
+    batch.capture(3)
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+    batch.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    errors = []
+    for ob, part, p in zip(obs_list, parts, plans):
+        c = ob.c
+        oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize,
+                              c.MaxTimestepsPerSubgrid, float(c.ScaleArcSec), c.WLayerCount, c.WStep)
+        meta = np.ascontiguousarray(part.subgrids[:: max(1, part.total // 24)])
+        with p9idg.Plan(c, kernel=args.kernel) as q:
+            q.set_observation(ob.uvw, ob.freq, meta)
+            q.upload_vis(ob.vis)
+            q.exec_grid(p9idg.GRID_WSTACK)
+            layers = q.download_grid(p9idg.GRID_WSTACK)
+        ref = O.gridw(oc, meta, ob.vis.astype(np.complex128), ob.uvw, ob.freq)
+        errors.append({"instance": ob.name, "grid": c.GridSize, "subgrid": c.SubgridSize,
+                       "gridded_visibilities": int(part.subgrids["timeSampleCount"].sum()) * len(ob.freq),
+                       "gridw_layers_max_rel": float(np.abs(layers - ref).max() / np.abs(ref).max())})
+    batch.close()
+    for p in plans:
+        p.close()
+    out = {
+        "metric": "Mvis/s grid+degrid, 8 batched uneven observations (one persistent launch per subgrid size, hipGraph)",
+        "value": n_vis / (ms / 1e3) / 1e6,
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.kernel],
+        "data": "synthetic (seeded; see p9idg/observation.py 'batch')",
+        "config": {"workload": "batch: 8 seeded observations, 512-2048 px images, subgrids 24/32/48",
+                   "kernel": args.kernel, "gridded_visibilities": int(n_vis)},
+        "stages": {"gridder_ms": kt["grid_ms"], "degridder_ms": kt["degrid_ms"]},
+        "error_vs_oracle": errors,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +274,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--workload", default="large", choices=["large", "batch"])
     ap.add_argument("--full-accuracy", action="store_true",
                     help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
@@ -204,6 +284,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return run_batch(args)
 
     import p9idg
     from p9idg import observation

@@ -318,7 +318,8 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int x = 16 * h + 8 * grp + e;
-                const float2 A = cis_f64(fma(ca, s_l[x], cc * s_n1[x]));
+                // padded columns (x >= S) meet zero pixels: keep them finite
+                const float2 A = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
                 P9_SPLIT(P, A.x, Arh[h], Arl[h], e);
                 P9_SPLIT(P, A.y, Aih[h], Ail[h], e);
             }
@@ -453,7 +454,8 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int x = 16 * h + 8 * grp + e;
-                const float2 A = cis_f64(fma(ca, s_l[x], cc * s_n1[x]));
+                // padded columns (x >= S) meet zero pixels: keep them finite
+                const float2 A = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
                 P9_SPLIT(P, A.x, Arh[h], Arl[h], e);
                 P9_SPLIT(P, A.y, Aih[h], Ail[h], e);
             }
