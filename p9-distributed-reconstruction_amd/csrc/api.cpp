@@ -86,6 +86,15 @@ struct p9idg_plan {
     bool capturing = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // Zero state of the gridder targets.  A buffer is "clean" when it is zero
+    // outside the footprint of the last gridding into it (dirty_fp: which
+    // footprint, -1 = none); otherwise (unknown contents) it is fully zeroed.
+    // The footprints (per-layer bounding boxes of the subgrids, per mode)
+    // live on the device.
+    bool grid_clean = false, layers_clean = false, model_vis_clean = false;
+    int32_t grid_dirty_fp = -1, layers_dirty_fp = -1;
+    Box* d_fp[3] = {nullptr, nullptr, nullptr};  // per grid mode
+    int32_t n_fp[3] = {0, 0, 0};
     int32_t use_queue = 1;
     int32_t grid_chunk = 2048;
     int32_t degrid_chunk = 2048;
@@ -157,6 +166,66 @@ size_t vis_elems(const p9idg_plan* p) { return (size_t)p->B * p->T * p->C; }
 
 int ensure_layers(p9idg_plan* p) {
     return dev_alloc(&p->d_layers, grid_elems(p) * (size_t)p->c.wlayer_count);
+}
+
+void mark_unknown(p9idg_plan* p) {
+    p->grid_clean = p->layers_clean = p->model_vis_clean = false;
+}
+
+// Per-mode footprints: per-layer bounding boxes of where the adder writes
+// (Adder.cs:30-42, AdderWStack.cs:29-70).
+int build_footprints(p9idg_plan* p, const p9idg_subgrid* meta, int64_t nsub) {
+    const int S = p->c.subgrid_size, G = p->c.grid_size, L = p->c.wlayer_count;
+    for (int mode = 0; mode < 3; mode++) {
+        const int nl = mode == P9IDG_GRID_WSTACK ? L : 1;
+        std::vector<Box> box(nl, Box{0, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN});
+        for (int l = 0; l < nl; l++) box[l].layer = l;
+        for (int64_t i = 0; i < nsub; i++) {
+            int X = meta[i].u_pixel, Y = meta[i].v_pixel, W = meta[i].w_lambda, layer = 0;
+            if (mode == P9IDG_GRID_PLAIN) {
+                if (!(X >= 0 && X < G - S && Y >= 0 && Y < G - S)) continue;
+            } else {
+                if (W < 0) {
+                    X = G - X - S + 1;
+                    Y = G - Y - S + 1;
+                    W = -W - 1;
+                }
+                if (!(X >= 1 && X < G - S && Y >= 1 && Y < G - S)) continue;
+                if (mode == P9IDG_GRID_WSTACK) layer = W;
+            }
+            if (layer < 0 || layer >= nl) continue;
+            Box& b = box[layer];
+            b.y0 = std::min(b.y0, Y);
+            b.y1 = std::max(b.y1, Y + S);
+            b.x0 = std::min(b.x0, X);
+            b.x1 = std::max(b.x1, X + S);
+        }
+        std::vector<Box> used;
+        for (const Box& b : box)
+            if (b.y1 > b.y0) used.push_back(b);
+        dev_free(p->d_fp[mode]);
+        p->n_fp[mode] = (int32_t)used.size();
+        if (used.empty()) continue;
+        if (int st = dev_alloc(&p->d_fp[mode], used.size())) return st;
+        P9_HIP(hipMemcpy(p->d_fp[mode], used.data(), used.size() * sizeof(Box), hipMemcpyHostToDevice));
+    }
+    return P9IDG_OK;
+}
+
+// Brings the gridder target of `mode` to all-zero: the previous footprint
+// when the contents are known, the whole buffer otherwise (and always while
+// capturing, since a graph may be replayed after other work on the buffer).
+int zero_target(p9idg_plan* p, int mode, float2* target, size_t n) {
+    bool& clean = mode == P9IDG_GRID_WSTACK ? p->layers_clean : p->grid_clean;
+    int32_t& fp = mode == P9IDG_GRID_WSTACK ? p->layers_dirty_fp : p->grid_dirty_fp;
+    if (p->capturing || !clean) {
+        P9_HIP(launch_fill_zero(target, n, p->stream));
+    } else if (fp >= 0) {
+        P9_HIP(launch_zero_boxes(target, p->d_fp[fp], p->n_fp[fp], p->c.grid_size, p->stream));
+    }
+    clean = !p->capturing;
+    fp = mode;
+    return P9IDG_OK;
 }
 
 int ensure_fft_one(p9idg_plan* p) {
@@ -340,7 +409,7 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     a.items = p->d_items_grid;
     a.nitems = p->nitems_grid;
     if (int st_ = record(p, 0)) return st_;
-    P9_HIP(launch_fill_zero(target, n, p->stream));
+    if (int st = zero_target(p, mode, target, n)) return st;
     if (int st_ = record(p, 1)) return st_;
     if (int st = span_begin(p, 0)) return st;
     hipError_t e;
@@ -363,7 +432,10 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     KernelArgs a = make_args(p, mode);
     a.items = p->d_items_degrid;
     a.nitems = p->nitems_degrid;
-    P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));  // Gridder.cs:96
+    // Gridder.cs:96: uncovered timesteps stay 0; only the degridder writes
+    // this buffer, and always the same (covered) entries
+    if (p->capturing || !p->model_vis_clean) P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));
+    p->model_vis_clean = !p->capturing;
     if (int st_ = record(p, 3)) return st_;
     if (int st = span_begin(p, 1)) return st;
     hipError_t e;
@@ -387,12 +459,14 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
     if (int st_ = record(p, 5)) return st_;
     if (mode == P9IDG_GRID_WSTACK) {
         if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
+        p->layers_clean = false;
         if (int st = ensure_fft_layers(p)) return st;
         P9_FFT(hipfftExecC2C(p->fft_layers, reinterpret_cast<hipfftComplex*>(p->d_layers),
                              reinterpret_cast<hipfftComplex*>(p->d_layers), HIPFFT_BACKWARD));
         P9_HIP(launch_layers_to_image(p->d_layers, p->c.wlayer_count, G, inv_n, 1, p->d_image,
                                       p->stream));
     } else {
+        p->grid_clean = false;
         if (int st = ensure_fft_one(p)) return st;
         P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_grid),
                              reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
@@ -606,6 +680,7 @@ int p9idg_destroy(p9idg_plan* p) {
     dev_free(p->d_flags);
     dev_free(p->d_counter);
     dev_free(p->d_resid);
+    for (auto& f : p->d_fp) dev_free(f);
     drop_graph(p);
     dev_free(p->d_items_grid);
     dev_free(p->d_items_degrid);
@@ -729,6 +804,8 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
     p->gridded_vis = gridded;
     p->meta_host.assign(meta, meta + nsub);
     p->order_host = order;
+    mark_unknown(p);
+    if (int st = build_footprints(p, meta, nsub)) return st;
     if (int st = upload_items(p)) return st;
     p->have_obs = true;
     return P9IDG_OK;
@@ -782,6 +859,7 @@ static int image_f64_host(p9idg_plan* p, double* img_out) {
     DeviceGuard dg(p->device);
     if (int st = ensure_fft_one(p)) return st;
     if (int st = dev_alloc(&p->d_image64, grid_elems(p))) return st;
+    p->grid_clean = false;
     P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_grid),
                          reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
     P9_HIP(launch_grid_to_image_f64(p->d_grid, p->c.grid_size,
@@ -844,6 +922,7 @@ int p9idg_wstack_ifft_float(p9idg_plan* p, const double* layers, int32_t L, int6
     if (int st = ensure_layers(p)) return st;
     if (int st = ensure_fft_layers(p)) return st;
     const size_t n = grid_elems(p) * L;
+    p->layers_clean = false;
     if (int st = upload_c128(p, layers, p->d_layers, n)) return st;
     P9_FFT(hipfftExecC2C(p->fft_layers, reinterpret_cast<hipfftComplex*>(p->d_layers),
                          reinterpret_cast<hipfftComplex*>(p->d_layers), HIPFFT_BACKWARD));
@@ -876,6 +955,7 @@ int p9idg_fft_backward(p9idg_plan* p, const double* grid, int64_t vis_count, dou
     DeviceGuard dg(p->device);
     if (int st = ensure_fft_one(p)) return st;
     if (int st = dev_alloc(&p->d_image64, grid_elems(p))) return st;
+    p->grid_clean = false;
     if (int st = upload_c128(p, grid, p->d_grid, grid_elems(p))) return st;
     P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_grid),
                          reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
@@ -969,6 +1049,7 @@ int p9idg_capture_cycle(p9idg_plan* p, int32_t grid_mode) {
     p->capturing = true;
     int st = cycle_step(p, grid_mode);
     p->capturing = false;
+    mark_unknown(p);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(p->stream, &g);
     if (st) {
@@ -1085,6 +1166,7 @@ int p9idg_device_buffers(p9idg_plan* p, void** vis, void** grid, void** model_gr
     if (int st = check_plan(p)) return st;
     if (vis) *vis = p->d_vis;
     if (grid) *grid = p->d_grid;
+    mark_unknown(p);  // the caller may write the buffers
     if (model_grid) *model_grid = p->d_model_grid;
     if (model_vis) *model_vis = p->d_model_vis;
     if (image) *image = p->d_image;
@@ -1247,6 +1329,7 @@ void batch_free_groups(p9idg_batch* b) {
 // Zero fills and one persistent launch per (subgrid size, kernel) group.
 int batch_run(p9idg_batch* b, int what, bool capturing) {
     if (!b->prepared) return fail(P9IDG_ERR_STATE, "batch not prepared (p9idg_batch_prepare)");
+    for (p9idg_plan* p : b->plans) mark_unknown(p);
     if (!capturing) P9_HIP(hipEventRecord(b->ev[0], b->stream));
     if (what & 1) {
         for (p9idg_plan* p : b->plans) {
@@ -1326,6 +1409,7 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
     b->degrid_mode = degrid_mode;
     for (int i = 0; i < (int)b->plans.size(); i++) {
         p9idg_plan* p = b->plans[i];
+        mark_unknown(p);
         if (grid_mode == P9IDG_GRID_PLAIN && !p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
         if (grid_mode == P9IDG_GRID_WSTACK)
             if (int st = ensure_layers(p)) return st;

@@ -21,7 +21,7 @@
 //   Q0[y][v] = sum_x P[y][x] conj(A_v[x]),  Q1[y][v] = sum_x r_xy P[y][x] conj(A_v[x])
 //   vis_v = scale * sum_y conj(B_v[y]) (Q0 - 2 pi i c_v Q1)
 // Phases are evaluated in FP64 (coefficients and per-entry phase), reduced to
-// [-0.5, 0.5] turns and passed to the hardware sin/cos.  The GEMM operands of
+// [0, 1) turns and passed to the hardware sin/cos.  The GEMM operands of
 // the main terms are split x = hi + lo in BF16 (or FP16) and multiplied as
 // hi*hi + hi*lo + lo*hi with FP32 accumulation; the correction terms use the
 // hi parts only.  FP16 operands are normalised by powers of two (exact) per
@@ -33,38 +33,69 @@ namespace dev {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+// An MFMA operand of 8 16-bit values, two per dword (element 2q in the low
+// half of dword q), so pairs are converted with one v_cvt_pk instruction.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 struct PrecBF16 {
     typedef __bf16 T;
     typedef bf16x8 V;
     static constexpr bool kNormalise = false;
-    __device__ static __forceinline__ f32x16 mma(V a, V b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    __device__ static __forceinline__ f32x16 mma(u32x4 a, u32x4 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b), c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ uint32_t pack(float a, float b) {
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+    }
+    __device__ static __forceinline__ f32x2 unpack(uint32_t h) {
+        return __builtin_convertvector(__builtin_bit_cast(bf16x2, h), f32x2);
     }
 };
 struct PrecF16 {
     typedef _Float16 T;
     typedef f16x8 V;
     static constexpr bool kNormalise = true;
-    __device__ static __forceinline__ f32x16 mma(V a, V b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    __device__ static __forceinline__ f32x16 mma(u32x4 a, u32x4 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b), c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ uint32_t pack(float a, float b) {
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, f16x2));
+    }
+    __device__ static __forceinline__ f32x2 unpack(uint32_t h) {
+        return __builtin_convertvector(__builtin_bit_cast(f16x2, h), f32x2);
     }
 };
 
-// x = hi + lo with hi = round(x), lo = round(x - hi) in the operand type.
-#define P9_SPLIT(P, x, VH, VL, e)                         \
-    do {                                                  \
-        const float x_ = (x);                             \
-        const typename P::T h_ = (typename P::T)x_;       \
-        (VH)[e] = h_;                                     \
-        (VL)[e] = (typename P::T)(x_ - (float)h_);        \
+// (a, b) = hi + lo per value, both rounded to nearest in the operand type.
+struct HiLo {
+    uint32_t hi, lo;
+};
+template <class P>
+__device__ __forceinline__ HiLo split2(float a, float b) {
+    HiLo r;
+    r.hi = P::pack(a, b);
+    const f32x2 h = P::unpack(r.hi);
+    r.lo = P::pack(a - h.x, b - h.y);
+    return r;
+}
+// Writes the split of (a, b) into the operand dwords EH (hi) and EL (lo).
+#define P9_SPLIT2(P, a, b, EH, EL)               \
+    do {                                         \
+        const HiLo s_ = split2<P>((a), (b));     \
+        EH = s_.hi;                              \
+        EL = s_.lo;                              \
     } while (0)
+// Negates both 16-bit values of every dword (sign bits).
+__device__ __forceinline__ u32x4 neg(u32x4 v) { return v ^ (u32x4){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u}; }
 
 // exp(2 pi i ph) for ph in turns (FP64), reduced to [-0.5, 0.5] before the
 // FP32 hardware sin/cos.
 __device__ __forceinline__ float2 cis_f64(double ph) {
-    const float r = (float)(ph - rint(ph));
+    const float r = (float)__builtin_amdgcn_fract(ph);  // [0, 1) turns
     return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
 }
 
@@ -108,8 +139,6 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
     constexpr int NT = (S + 31) / 32;  // tiles per axis
     constexpr int WPK = 4 / (NT * NT); // waves sharing one tile's visibilities
     static_assert(NT * 32 >= S && NT <= 2, "subgrids up to 64");
-    typedef typename P::T T;
-    typedef typename P::V V;
     constexpr int ROUND = kThreads;
     __shared__ double s_a[ROUND], s_b[ROUND], s_c[ROUND];
     __shared__ float2 s_v[ROUND];
@@ -170,24 +199,27 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
         __syncthreads();
         const int nsteps = (min(ROUND, K - j0) + 15) / 16;
         for (int st = kpart; st < nsteps; st += WPK) {
-            V Lrh, Lrl, Lih, Lil, cLr, cLi, Arh, Arl, Aih, Ail;
+            u32x4 Lrh, Lrl, Lih, Lil, cLr, cLi, Arh, Arl, Aih, Ail;
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int j = st * 16 + grp * 8 + e;
-                const double ca = s_a[j], cb = s_b[j], cc = s_c[j];
-                const float2 v = s_v[j];
-                const float2 A = cis_f64(fma(ca, lcol, cc * ncol));
-                const float2 B = cis_f64(fma(cb, lrow, cc * nrow));
-                const float2 L = cmulf(v, B);
-                const float cf = (float)cc * cscale;
-                P9_SPLIT(P, L.x, Lrh, Lrl, e);
-                P9_SPLIT(P, L.y, Lih, Lil, e);
-                P9_SPLIT(P, A.x, Arh, Arl, e);
-                P9_SPLIT(P, A.y, Aih, Ail, e);
-                cLr[e] = (T)(cf * L.x);
-                cLi[e] = (T)(cf * L.y);
+            for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
+                float2 L[2], A[2];
+                float cf[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int j = st * 16 + grp * 8 + 2 * q + h;
+                    const double ca = s_a[j], cb = s_b[j], cc = s_c[j];
+                    A[h] = cis_f64(fma(ca, lcol, cc * ncol));
+                    L[h] = cmulf(s_v[j], cis_f64(fma(cb, lrow, cc * nrow)));
+                    cf[h] = (float)cc * cscale;
+                }
+                P9_SPLIT2(P, L[0].x, L[1].x, Lrh[q], Lrl[q]);
+                P9_SPLIT2(P, L[0].y, L[1].y, Lih[q], Lil[q]);
+                P9_SPLIT2(P, A[0].x, A[1].x, Arh[q], Arl[q]);
+                P9_SPLIT2(P, A[0].y, A[1].y, Aih[q], Ail[q]);
+                cLr[q] = P::pack(cf[0] * L[0].x, cf[1] * L[1].x);
+                cLi[q] = P::pack(cf[0] * L[0].y, cf[1] * L[1].y);
             }
-            const V nLih = -Lih, nLil = -Lil, ncLi = -cLi;
+            const u32x4 nLih = neg(Lih), nLil = neg(Lil), ncLi = neg(cLi);
             // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi)
             acc0r = P::mma(Lrh, Arh, acc0r);
             acc0i = P::mma(Lrh, Aih, acc0i);
@@ -242,7 +274,6 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 template <int S, class P>
 __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
     static_assert(S <= 32, "one 32x32 MFMA tile per subgrid");
-    typedef typename P::V V;
     __shared__ float2 s_tile[2][S * S];
     __shared__ double s_l[32], s_n1[32];
     __shared__ float s_red[4];
@@ -273,28 +304,33 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
 
     // A operand: pixels_copy[y][x] = subgrid[(y+S/2)%S][(x+S/2)%S] (Gridder.cs:101-117)
     // lane holds row y = li, k = x = 16 h + 8 grp + e
-    V Prh[2], Prl[2], Pih[2], Pil[2], nPrh[2], nPrl[2], Qr[2], Qi[2], nQr[2];
+    u32x4 Prh[2], Prl[2], Pih[2], Pil[2], nPrh[2], nPrl[2], Qr[2], Qi[2], nQr[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int y = li, x = 16 * h + 8 * grp + e;
-            float2 pv = make_float2(0.f, 0.f);
-            float rr = 0.f;
-            if (y < S && x < S) {
-                pv = res[((y + S / 2) % S) * S + (x + S / 2) % S];
-                pv.x *= pscale;
-                pv.y *= pscale;
-                rr = (float)residual_n(x, y, S, a.image_size) * rscale;
+        for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
+            float2 pv[2];
+            float rr[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int y = li, x = 16 * h + 8 * grp + 2 * q + u;
+                pv[u] = make_float2(0.f, 0.f);
+                rr[u] = 0.f;
+                if (y < S && x < S) {
+                    pv[u] = res[((y + S / 2) % S) * S + (x + S / 2) % S];
+                    pv[u].x *= pscale;
+                    pv[u].y *= pscale;
+                    rr[u] = (float)residual_n(x, y, S, a.image_size) * rscale;
+                }
             }
-            P9_SPLIT(P, pv.x, Prh[h], Prl[h], e);
-            P9_SPLIT(P, pv.y, Pih[h], Pil[h], e);
-            Qr[h][e] = (typename P::T)(rr * pv.x);
-            Qi[h][e] = (typename P::T)(rr * pv.y);
+            P9_SPLIT2(P, pv[0].x, pv[1].x, Prh[h][q], Prl[h][q]);
+            P9_SPLIT2(P, pv[0].y, pv[1].y, Pih[h][q], Pil[h][q]);
+            Qr[h][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
+            Qi[h][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
-        nPrh[h] = -Prh[h];
-        nPrl[h] = -Prl[h];
-        nQr[h] = -Qr[h];
+        nPrh[h] = neg(Prh[h]);
+        nPrl[h] = neg(Prl[h]);
+        nQr[h] = neg(Qr[h]);
     }
     // this lane's rows of the output tile and their (m, m^2/2)
     const float out_scale = a.degrid_scale / pscale;
@@ -312,16 +348,20 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
         const double cc = (g.w_off - k * p[2]) * kInvTwoPi;
 
         // B operand: A_v[x] for this lane's vis (column) and x = 16 h + 8 grp + e
-        V Arh[2], Arl[2], Aih[2], Ail[2];
+        u32x4 Arh[2], Arl[2], Aih[2], Ail[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int x = 16 * h + 8 * grp + e;
-                // padded columns (x >= S) meet zero pixels: keep them finite
-                const float2 A = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
-                P9_SPLIT(P, A.x, Arh[h], Arl[h], e);
-                P9_SPLIT(P, A.y, Aih[h], Ail[h], e);
+            for (int q = 0; q < 4; q++) {
+                float2 A[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int x = 16 * h + 8 * grp + 2 * q + u;
+                    // padded columns (x >= S) meet zero pixels: keep them finite
+                    A[u] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
+                }
+                P9_SPLIT2(P, A[0].x, A[1].x, Arh[h][q], Arl[h][q]);
+                P9_SPLIT2(P, A[0].y, A[1].y, Aih[h][q], Ail[h][q]);
             }
         }
         f32x16 q0r = {}, q0i = {}, q1r = {}, q1i = {};
@@ -373,10 +413,9 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
 template <int S, class P>
 __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int sidx, int k0, int k1) {
     static_assert(S > 32 && S <= 64, "two 32-wide tiles per axis");
-    typedef typename P::V V;
     constexpr int RT = 2, KS = 4, NA = 6;  // row tiles, k-steps, operand arrays
     __shared__ float2 s_tile[2][S * S];
-    __shared__ V s_op[RT * KS * NA * 64];
+    __shared__ u32x4 s_op[RT * KS * NA * 64];
     __shared__ double s_l[64], s_n1[64];
     __shared__ float s_red[4];
 
@@ -408,24 +447,29 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
     for (int q = tid; q < RT * KS * 64; q += kThreads) {
         const int ln = q & 63, h = (q >> 6) % KS, rt = q / (64 * KS);
         const int y = rt * 32 + (ln & 31);
-        V prh, prl, pih, pil, qr, qi;
+        u32x4 prh, prl, pih, pil, qr, qi;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int x = 16 * h + 8 * (ln >> 5) + e;
-            float2 pv = make_float2(0.f, 0.f);
-            float rr = 0.f;
-            if (y < S && x < S) {
-                pv = res[((y + S / 2) % S) * S + (x + S / 2) % S];
-                pv.x *= pscale;
-                pv.y *= pscale;
-                rr = (float)residual_n(x, y, S, a.image_size) * rscale;
+        for (int w = 0; w < 4; w++) {  // elements 2w, 2w+1
+            float2 pv[2];
+            float rr[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int x = 16 * h + 8 * (ln >> 5) + 2 * w + u;
+                pv[u] = make_float2(0.f, 0.f);
+                rr[u] = 0.f;
+                if (y < S && x < S) {
+                    pv[u] = res[((y + S / 2) % S) * S + (x + S / 2) % S];
+                    pv[u].x *= pscale;
+                    pv[u].y *= pscale;
+                    rr[u] = (float)residual_n(x, y, S, a.image_size) * rscale;
+                }
             }
-            P9_SPLIT(P, pv.x, prh, prl, e);
-            P9_SPLIT(P, pv.y, pih, pil, e);
-            qr[e] = (typename P::T)(rr * pv.x);
-            qi[e] = (typename P::T)(rr * pv.y);
+            P9_SPLIT2(P, pv[0].x, pv[1].x, prh[w], prl[w]);
+            P9_SPLIT2(P, pv[0].y, pv[1].y, pih[w], pil[w]);
+            qr[w] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
+            qi[w] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
-        V* op = &s_op[((rt * KS + h) * NA) * 64 + ln];
+        u32x4* op = &s_op[((rt * KS + h) * NA) * 64 + ln];
         op[0 * 64] = prh;
         op[1 * 64] = prl;
         op[2 * 64] = pih;
@@ -448,16 +492,20 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
         const double cb = (g.v_off - k * p[1]) * kInvTwoPi;
         const double cc = (g.w_off - k * p[2]) * kInvTwoPi;
 
-        V Arh[KS], Arl[KS], Aih[KS], Ail[KS];
+        u32x4 Arh[KS], Arl[KS], Aih[KS], Ail[KS];
 #pragma unroll
         for (int h = 0; h < KS; h++) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int x = 16 * h + 8 * grp + e;
-                // padded columns (x >= S) meet zero pixels: keep them finite
-                const float2 A = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
-                P9_SPLIT(P, A.x, Arh[h], Arl[h], e);
-                P9_SPLIT(P, A.y, Aih[h], Ail[h], e);
+            for (int q = 0; q < 4; q++) {
+                float2 A[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int x = 16 * h + 8 * grp + 2 * q + u;
+                    // padded columns (x >= S) meet zero pixels: keep them finite
+                    A[u] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
+                }
+                P9_SPLIT2(P, A[0].x, A[1].x, Arh[h][q], Arl[h][q]);
+                P9_SPLIT2(P, A[0].y, A[1].y, Aih[h][q], Ail[h][q]);
             }
         }
         const float cf = (float)cc * q1_back;
@@ -467,10 +515,10 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
             f32x16 q0r = {}, q0i = {}, q1r = {}, q1i = {};
 #pragma unroll
             for (int h = 0; h < KS; h++) {
-                const V* op = &s_op[((rt * KS + h) * NA) * 64 + lane];
-                const V prh = op[0 * 64], prl = op[1 * 64], pih = op[2 * 64], pil = op[3 * 64];
-                const V qr = op[4 * 64], qi = op[5 * 64];
-                const V nAih = -Aih[h], nAil = -Ail[h];
+                const u32x4* op = &s_op[((rt * KS + h) * NA) * 64 + lane];
+                const u32x4 prh = op[0 * 64], prl = op[1 * 64], pih = op[2 * 64], pil = op[3 * 64];
+                const u32x4 qr = op[4 * 64], qi = op[5 * 64];
+                const u32x4 nAih = neg(Aih[h]), nAil = neg(Ail[h]);
                 // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai)
                 q0r = P::mma(prh, Arh[h], q0r);
                 q0i = P::mma(pih, Arh[h], q0i);

@@ -94,6 +94,18 @@ __global__ void residual_kernel(const float2* vis, const float2* model, const ui
     }
 }
 
+// Zeroes the (quadrant-swapped) image of each box: blockIdx.y is the box.
+__global__ void zero_boxes_kernel(float2* base, const Box* boxes, int G) {
+    const Box b = boxes[blockIdx.y];
+    const int w = b.x1 - b.x0;
+    const size_t n = (size_t)(b.y1 - b.y0) * (size_t)w;
+    float2* layer = base + (size_t)b.layer * G * G;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = b.y0 + (int)(i / w), x = b.x0 + (int)(i % w);
+        layer[(size_t)shifted(y, G) * G + shifted(x, G)] = make_float2(0.f, 0.f);
+    }
+}
+
 inline unsigned blocks_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
@@ -157,6 +169,11 @@ hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t
                            float2* out, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(dev::residual_kernel, dim3(dev::blocks_for(n)), dim3(256), 0, st, vis, model,
                        flags, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_zero_boxes(float2* base, const Box* boxes, int nboxes, int G, hipStream_t st) {
+    if (nboxes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dev::zero_boxes_kernel, dim3(64, nboxes), dim3(256), 0, st, base, boxes, G);
     return hipGetLastError();
 }
 hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st) {

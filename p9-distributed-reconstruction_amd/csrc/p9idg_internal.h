@@ -56,6 +56,12 @@ struct LaunchCfg {
     int32_t workgroups = 0; // persistent workgroups (0: device CUs x occupancy)
 };
 
+// Rectangle [y0, y1) x [x0, x1) of grid layer `layer` in the grid's own
+// (unshifted) coordinates; zero_boxes applies the quadrant swap.
+struct Box {
+    int32_t layer, y0, y1, x0, x1;
+};
+
 // Kernel launch parameters shared by host and device code.
 struct KernelArgs {
     const p9idg_subgrid* meta;
@@ -113,6 +119,7 @@ hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid
                                 hipStream_t st);
 hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t* flags,
                            float2* out, size_t n, hipStream_t st);
+hipError_t launch_zero_boxes(float2* base, const Box* boxes, int nboxes, int G, hipStream_t st);
 hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st);
 
 }  // namespace p9idg
