@@ -77,38 +77,41 @@ def cpu_baseline_and_error(args, ob, part, model_grid_c64, kernel):
     meta = part.subgrids
     vis128 = ob.vis.astype(np.complex128)
     model128 = model_grid_c64.astype(np.complex128)
-    # calibrate the oracle's rate on a small sample, then size the real one
-    probe = sample_subgrids(meta, C, 20000)
-    t = time.perf_counter()
-    sub = O.gridder_forward(oc, probe, ob.uvw, vis128, ob.freq)
-    O.gridder_backward(oc, probe, sub, ob.uvw, ob.freq)
-    rate = int(probe["timeSampleCount"].sum()) * C / max(time.perf_counter() - t, 1e-6)
-    target = max(20000, int(rate * args.cpu_seconds))
-    sample = sample_subgrids(meta, C, target)
-    n_vis = int(sample["timeSampleCount"].sum()) * C
-    log("cpu baseline: %d subgrids, %d visibilities, %d threads" % (len(sample), n_vis, threads))
-
-    t0 = time.perf_counter()
-    sub = O.gridder_forward(oc, sample, ob.uvw, vis128, ob.freq)
-    sub = O.subgrid_fft_forward(oc, sub)
-    layers_ref = O.adderw_add(oc, sample, sub)
-    t1 = time.perf_counter()
     g = np.array(model128, copy=True)
     O.shift(g)  # IDG.DeGridW shifts the grid first (IDG.cs:44)
-    sub = O.adderw_split(oc, sample, g)
-    sub = O.subgrid_fft_backward(oc, sub)
-    vis_ref = O.gridder_backward(oc, sample, sub, ob.uvw, ob.freq)
-    t2 = time.perf_counter()
-    del sub, g
+
+    def run(smp):
+        t0 = time.perf_counter()
+        sub = O.gridder_forward(oc, smp, ob.uvw, vis128, ob.freq)
+        sub = O.subgrid_fft_forward(oc, sub)
+        lay = O.adderw_add(oc, smp, sub)
+        t1 = time.perf_counter()
+        sub = O.adderw_split(oc, smp, g)
+        sub = O.subgrid_fft_backward(oc, sub)
+        v = O.gridder_backward(oc, smp, sub, ob.uvw, ob.freq)
+        t2 = time.perf_counter()
+        return lay, v, t1 - t0, t2 - t1
+
+    # calibrate the oracle's rate on about 1/64 of the workload (whole
+    # pipeline, both directions), then size the timed sample to the target
+    probe = sample_subgrids(meta, C, int(meta["timeSampleCount"].sum()) * C // 64)
+    _, _, pa, pb = run(probe)
+    del _
+    rate = int(probe["timeSampleCount"].sum()) * C / max(pa + pb, 1e-6)
+    sample = sample_subgrids(meta, C, max(20000, int(rate * args.cpu_seconds)))
+    n_vis = int(sample["timeSampleCount"].sum()) * C
+    log("cpu baseline: %d subgrids, %d visibilities, %d threads" % (len(sample), n_vis, threads))
+    layers_ref, vis_ref, tg, td = run(sample)
+    del g
     for k in range(layers_ref.shape[0]):
         O.shift(layers_ref[k])  # IDG.GridW shifts each layer (IDG.cs:37)
     cpu = {
-        "value": n_vis / (t2 - t0) / 1e6,
+        "value": n_vis / (tg + td) / 1e6,
         "unit": "Mvis/s",
         "cores": threads,
         "kind": "port",
         "sample": "every %d-th subgrid of the workload: %d subgrids, %d visibilities; FP64 grid %.1f s + degrid %.1f s"
-        % (max(1, len(meta) // max(len(sample), 1)), len(sample), n_vis, t1 - t0, t2 - t1),
+        % (max(1, len(meta) // max(len(sample), 1)), len(sample), n_vis, tg, td),
     }
 
     # GPU on the same sample
@@ -265,12 +268,33 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_probe(gib):
+    """A plain copy kernel of known size, profiled beside the gridder to turn
+    FETCH_SIZE / WRITE_SIZE counter readings into bytes."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy probe: %d bytes x 3 launches" % (4 * n))
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kernel", default="fp32", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--kernel", default="mfma_bf16", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--launch", default=None, metavar="QUEUE,GRID_CHUNK,DEGRID_CHUNK,WORKGROUPS",
+                    help="fixed launch shape instead of the autotuner's (e.g. 1,0,0,256)")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="first run a plain device-to-device copy of this many GiB (3 times): the "
+                         "reference kernel for scaling HBM byte counters")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
@@ -285,6 +309,8 @@ def main():
               "(requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     if args.workload == "batch":
         return run_batch(args)
 
@@ -309,8 +335,11 @@ def main():
     plan.sync()
     n_vis = plan.gridded_visibilities
     log("gridded visibilities: %d of %d" % (n_vis, ob.vis.size))
+    if args.launch:
+        q, gc, dc, wg = (int(v) for v in args.launch.split(","))
+        plan.set_launch(q, gc, dc, wg)
     tuning = plan.get_launch()
-    if not args.no_autotune:
+    if not args.no_autotune and not args.launch:
         tuning = plan.autotune(reps=3)
         log("autotuned launch shape: %s" % tuning)
 

@@ -149,9 +149,10 @@ int p9idg_get_launch(const p9idg_plan* plan, int32_t* queue, int32_t* grid_chunk
                      int32_t* degrid_chunk, int32_t* workgroups);
 /* Chooses the launch shape (queue or not, visibilities per work item for
  * each direction, persistent workgroups) by timing candidates on the plan's
- * observation with HIP events (median of reps runs each); keeps the fastest
- * and returns its gridder / degridder times.  Overwrites the grid and model
- * visibility buffers. */
+ * observation with HIP events (median of reps runs each), in the w-stacked
+ * modes when the constants have more than one w-layer and the plain modes
+ * otherwise; keeps the fastest and returns its gridder / degridder kernel
+ * times.  Overwrites the grid / layer and model visibility buffers. */
 int p9idg_autotune(p9idg_plan* plan, int32_t reps, float* grid_ms, float* degrid_ms);
 /* Uploads uvw, frequencies and the partition to HBM (once per observation). */
 int p9idg_set_observation(p9idg_plan* plan, const double* uvw, int32_t nr_baselines,

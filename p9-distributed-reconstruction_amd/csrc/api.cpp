@@ -49,6 +49,7 @@ struct p9idg_plan {
     int32_t B = 0, T = 0, C = 0;
     int64_t nsub = 0;
     bool plain_adder_ok = true;  // every subgrid passes Adder.cs:30-31
+    bool chan_uniform = false;   // wavenumbers equally spaced (to 1e-12 relative)
     int64_t gridded_vis = 0;
 
     // device buffers
@@ -385,6 +386,7 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
     const int S = p->c.subgrid_size;
     a.degrid_scale = 1.0f / (float)(S * S);                      // Gridder.cs:151
     a.mode = mode;
+    a.chan_rec = p->chan_uniform && p->C % 8 == 0;
     a.counter = p->d_counter;
     return a;
 }
@@ -783,6 +785,11 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
     // MathFunctions.cs:69-78: k = 2 pi f / c, (2.0f promoted to double)
     std::vector<double> k(C);
     for (int i = 0; i < C; i++) k[i] = ((2.0 * M_PI) * freq[i]) / 299792458.0;
+    bool uniform = C >= 2;
+    if (uniform) {
+        const double dk = (k[C - 1] - k[0]) / (C - 1), tol = 1e-12 * std::fabs(k[C - 1]);
+        for (int i = 0; i < C; i++) uniform = uniform && std::fabs(k[i] - (k[0] + i * dk)) <= tol;
+    }
     // schedule: most visibilities first
     std::vector<int32_t> order(nsub);
     std::iota(order.begin(), order.end(), 0);
@@ -796,6 +803,7 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
         P9_HIP(hipMemcpyAsync(p->d_order, order.data(), nsub * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
     }
     P9_HIP(hipStreamSynchronize(p->stream));
+    p->chan_uniform = uniform;
     p->B = B;
     p->T = T;
     p->C = C;
@@ -1076,10 +1084,14 @@ int p9idg_launch_cycle(p9idg_plan* p) {
 
 // Times one gridder (dir 0) or degridder (dir 1) launch with the plan's
 // current launch shape; median of `reps` runs.
+// Median kernel time of one direction over `reps` launches, in the mode a
+// major cycle runs: w-stacked when the constants have w-layers.
 static int time_direction(p9idg_plan* p, int dir, int reps, float* ms_out) {
+    const bool wstack = p->c.wlayer_count > 1;
     std::vector<float> t;
     for (int r = 0; r < reps; r++) {
-        int st = dir == 0 ? do_exec_grid(p, P9IDG_GRID_WSUM) : do_exec_degrid(p, P9IDG_DEGRID_WSTACK);
+        int st = dir == 0 ? do_exec_grid(p, wstack ? P9IDG_GRID_WSTACK : P9IDG_GRID_PLAIN)
+                          : do_exec_degrid(p, wstack ? P9IDG_DEGRID_WSTACK : P9IDG_DEGRID_PLAIN);
         if (st) return st;
         P9_HIP(hipStreamSynchronize(p->stream));
         float ms = 0.f;

@@ -150,6 +150,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
     const int li = lane & 31, grp = lane >> 5;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
+    const bool chan_rec = a.chan_rec != 0;
 
     for (int p = tid; p < S * S; p += kThreads) s_tile[0][p] = make_float2(0.f, 0.f);
     __syncthreads();
@@ -200,24 +201,59 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
         const int nsteps = (min(ROUND, K - j0) + 15) / 16;
         for (int st = kpart; st < nsteps; st += WPK) {
             u32x4 Lrh, Lrl, Lih, Lil, cLr, cLi, Arh, Arl, Aih, Ail;
+            const int jg = st * 16 + grp * 8;  // this lane group's 8 visibilities
+            if (chan_rec) {
+                // one timestep's 8 channels: phasors at the first channel times
+                // the channel-step phasor (FP64 phases, FP32 products)
+                const double ca = s_a[jg], cb = s_b[jg], cc = s_c[jg];
+                const double da = s_a[jg + 1] - ca, db = s_b[jg + 1] - cb, dc = s_c[jg + 1] - cc;
+                float2 A = cis_f64(fma(ca, lcol, cc * ncol));
+                float2 B = cis_f64(fma(cb, lrow, cc * nrow));
+                const float2 RA = cis_f64(fma(da, lcol, dc * ncol));
+                const float2 RB = cis_f64(fma(db, lrow, dc * nrow));
+                const float c0 = (float)cc * cscale, d0 = (float)dc * cscale;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
-                float2 L[2], A[2];
-                float cf[2];
+                for (int q = 0; q < 4; q++) {
+                    float2 L[2], Aq[2];
+                    float cf[2];
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int j = st * 16 + grp * 8 + 2 * q + h;
-                    const double ca = s_a[j], cb = s_b[j], cc = s_c[j];
-                    A[h] = cis_f64(fma(ca, lcol, cc * ncol));
-                    L[h] = cmulf(s_v[j], cis_f64(fma(cb, lrow, cc * nrow)));
-                    cf[h] = (float)cc * cscale;
+                    for (int h = 0; h < 2; h++) {
+                        const int e = 2 * q + h;
+                        Aq[h] = A;
+                        L[h] = cmulf(s_v[jg + e], B);
+                        cf[h] = fmaf((float)e, d0, c0);
+                        if (e < 7) {
+                            A = cmulf(A, RA);
+                            B = cmulf(B, RB);
+                        }
+                    }
+                    P9_SPLIT2(P, L[0].x, L[1].x, Lrh[q], Lrl[q]);
+                    P9_SPLIT2(P, L[0].y, L[1].y, Lih[q], Lil[q]);
+                    P9_SPLIT2(P, Aq[0].x, Aq[1].x, Arh[q], Arl[q]);
+                    P9_SPLIT2(P, Aq[0].y, Aq[1].y, Aih[q], Ail[q]);
+                    cLr[q] = P::pack(cf[0] * L[0].x, cf[1] * L[1].x);
+                    cLi[q] = P::pack(cf[0] * L[0].y, cf[1] * L[1].y);
                 }
-                P9_SPLIT2(P, L[0].x, L[1].x, Lrh[q], Lrl[q]);
-                P9_SPLIT2(P, L[0].y, L[1].y, Lih[q], Lil[q]);
-                P9_SPLIT2(P, A[0].x, A[1].x, Arh[q], Arl[q]);
-                P9_SPLIT2(P, A[0].y, A[1].y, Aih[q], Ail[q]);
-                cLr[q] = P::pack(cf[0] * L[0].x, cf[1] * L[1].x);
-                cLi[q] = P::pack(cf[0] * L[0].y, cf[1] * L[1].y);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
+                    float2 L[2], A[2];
+                    float cf[2];
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int j = jg + 2 * q + h;
+                        const double ca = s_a[j], cb = s_b[j], cc = s_c[j];
+                        A[h] = cis_f64(fma(ca, lcol, cc * ncol));
+                        L[h] = cmulf(s_v[j], cis_f64(fma(cb, lrow, cc * nrow)));
+                        cf[h] = (float)cc * cscale;
+                    }
+                    P9_SPLIT2(P, L[0].x, L[1].x, Lrh[q], Lrl[q]);
+                    P9_SPLIT2(P, L[0].y, L[1].y, Lih[q], Lil[q]);
+                    P9_SPLIT2(P, A[0].x, A[1].x, Arh[q], Arl[q]);
+                    P9_SPLIT2(P, A[0].y, A[1].y, Aih[q], Ail[q]);
+                    cLr[q] = P::pack(cf[0] * L[0].x, cf[1] * L[1].x);
+                    cLi[q] = P::pack(cf[0] * L[0].y, cf[1] * L[1].y);
+                }
             }
             const u32x4 nLih = neg(Lih), nLil = neg(Lil), ncLi = neg(cLi);
             // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi)
@@ -304,7 +340,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
 
     // A operand: pixels_copy[y][x] = subgrid[(y+S/2)%S][(x+S/2)%S] (Gridder.cs:101-117)
     // lane holds row y = li, k = x = 16 h + 8 grp + e
-    u32x4 Prh[2], Prl[2], Pih[2], Pil[2], nPrh[2], nPrl[2], Qr[2], Qi[2], nQr[2];
+    u32x4 Prh[2], Prl[2], Pih[2], Pil[2], Qr[2], Qi[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
 #pragma unroll
@@ -328,13 +364,118 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
             Qr[h][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
             Qi[h][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
-        nPrh[h] = neg(Prh[h]);
-        nPrl[h] = neg(Prl[h]);
-        nQr[h] = neg(Qr[h]);
     }
-    // this lane's rows of the output tile and their (m, m^2/2)
     const float out_scale = a.degrid_scale / pscale;
     const float q1_back = kTwoPiF / rscale;
+
+    // Q0 = P conj(A), Q1 = (r P) conj(A) for one tile of 32 visibilities
+    auto tile_mma = [&](const u32x4 (&Arh)[2], const u32x4 (&Arl)[2], const u32x4 (&Aih)[2],
+                        const u32x4 (&Ail)[2], f32x16& q0r, f32x16& q0i, f32x16& q1r, f32x16& q1i) {
+        q0r = f32x16{};
+        q0i = f32x16{};
+        q1r = f32x16{};
+        q1i = f32x16{};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const u32x4 nAih = neg(Aih[h]), nAil = neg(Ail[h]);
+            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai)
+            q0r = P::mma(Prh[h], Arh[h], q0r);
+            q0i = P::mma(Pih[h], Arh[h], q0i);
+            q1r = P::mma(Qr[h], Arh[h], q1r);
+            q1i = P::mma(Qi[h], Arh[h], q1i);
+            q0r = P::mma(Prh[h], Arl[h], q0r);
+            q0i = P::mma(Pih[h], Arl[h], q0i);
+            q1r = P::mma(Qi[h], Aih[h], q1r);
+            q1i = P::mma(Qr[h], nAih, q1i);
+            q0r = P::mma(Prl[h], Arh[h], q0r);
+            q0i = P::mma(Pil[h], Arh[h], q0i);
+            q0r = P::mma(Pih[h], Aih[h], q0r);
+            q0i = P::mma(Prh[h], nAih, q0i);
+            q0r = P::mma(Pih[h], Ail[h], q0r);
+            q0i = P::mma(Prh[h], nAil, q0i);
+            q0r = P::mma(Pil[h], Aih[h], q0r);
+            q0i = P::mma(Prl[h], nAih, q0i);
+        }
+    };
+
+    if (a.chan_rec) {
+        // Channel recurrence.  Column li of every tile is one timestep and
+        // consecutive tiles are consecutive channels, so each lane steps its
+        // 16 x-phasors and 16 y-phasors by one multiplication per tile.  A
+        // super-tile is 32 timesteps x C channels; the (super-tile, channel)
+        // sequence is cut into 4 equal contiguous parts, one per wave, and the
+        // phasors are evaluated directly at the start of every run and every
+        // 8 channels.
+        const int tlo = k0 / C, thi = (K + C - 1) / C;
+        const int nst = (thi - tlo + 31) / 32;
+        const int64_t F = (int64_t)nst * C;
+        int64_t f = F * wave / 4;
+        const int64_t f_end = F * (wave + 1) / 4;
+        while (f < f_end) {
+            const int st = (int)(f / C), ch0 = (int)(f % C);
+            const int run = (int)min((int64_t)(C - ch0), f_end - f);
+            const int tau = tlo + st * 32 + li;  // this lane's timestep in the subgrid
+            const double* p = a.uvw + ((size_t)g.bl * a.T + g.t0 + min(tau, g.nt - 1)) * 3;
+            const double u = p[0], v = p[1], w = p[2];
+#pragma unroll 1
+            for (int cb = 0; cb < run; cb += 8) {
+                const int ch = ch0 + cb;
+                const double k = a.wavenumbers[ch];
+                const double dk = a.wavenumbers[ch + 1 < C ? ch + 1 : ch] - k;
+                const double ca = (g.u_off - k * u) * kInvTwoPi, cbv = (g.v_off - k * v) * kInvTwoPi;
+                const double cc = (g.w_off - k * w) * kInvTwoPi;
+                const double da = -dk * u * kInvTwoPi, db = -dk * v * kInvTwoPi, dc = -dk * w * kInvTwoPi;
+                float2 As[16], RA[16], Bs[16], RB[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const int x = 16 * (i / 8) + 8 * grp + (i % 8);
+                    const int y = 8 * (i / 4) + 4 * grp + (i % 4);
+                    // padded columns (x >= S) meet zero pixels: keep them finite
+                    As[i] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
+                    RA[i] = x < S ? cis_f64(fma(da, s_l[x], dc * s_n1[x])) : make_float2(0.f, 0.f);
+                    Bs[i] = y < S ? cis_f64(fma(cbv, s_l[y], cc * s_n1[y])) : make_float2(0.f, 0.f);
+                    RB[i] = y < S ? cis_f64(fma(db, s_l[y], dc * s_n1[y])) : make_float2(0.f, 0.f);
+                }
+                const float cf0 = (float)cc * q1_back, dcf = (float)dc * q1_back;
+                const int ne = min(8, run - cb);
+#pragma unroll 1
+                for (int e = 0; e < ne; e++) {
+                    u32x4 Arh[2], Arl[2], Aih[2], Ail[2];
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            P9_SPLIT2(P, As[8 * h + 2 * q].x, As[8 * h + 2 * q + 1].x, Arh[h][q], Arl[h][q]);
+                            P9_SPLIT2(P, As[8 * h + 2 * q].y, As[8 * h + 2 * q + 1].y, Aih[h][q], Ail[h][q]);
+                        }
+                    }
+                    f32x16 q0r, q0i, q1r, q1i;
+                    tile_mma(Arh, Arl, Aih, Ail, q0r, q0i, q1r, q1i);
+                    const float cf = fmaf((float)e, dcf, cf0);
+                    float sr = 0.f, si = 0.f;
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const float zr = q0r[r] + cf * q1i[r];
+                        const float zi = q0i[r] - cf * q1r[r];
+                        sr = fmaf(Bs[r].x, zr, fmaf(Bs[r].y, zi, sr));
+                        si = fmaf(Bs[r].x, zi, fmaf(-Bs[r].y, zr, si));
+                    }
+                    sr += __shfl_xor(sr, 32);
+                    si += __shfl_xor(si, 32);
+                    const int j = tau * C + ch + e;
+                    if (grp == 0 && tau < thi && j >= k0 && j < K)
+                        a.vis_out[vbase + j] = make_float2(sr * out_scale, si * out_scale);
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        As[i] = cmulf(As[i], RA[i]);
+                        Bs[i] = cmulf(Bs[i], RB[i]);
+                    }
+                }
+            }
+            f += run;
+        }
+        return;
+    }
 
     const int ntiles = (K - k0 + 31) / 32;
     for (int vt = wave; vt < ntiles; vt += 4) {
@@ -364,27 +505,8 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 P9_SPLIT2(P, A[0].y, A[1].y, Aih[h][q], Ail[h][q]);
             }
         }
-        f32x16 q0r = {}, q0i = {}, q1r = {}, q1i = {};
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar - Pr Ai
-            q0r = P::mma(Prh[h], Arh[h], q0r);
-            q0i = P::mma(Pih[h], Arh[h], q0i);
-            q1r = P::mma(Qr[h], Arh[h], q1r);
-            q1i = P::mma(Qi[h], Arh[h], q1i);
-            q0r = P::mma(Prh[h], Arl[h], q0r);
-            q0i = P::mma(Pih[h], Arl[h], q0i);
-            q1r = P::mma(Qi[h], Aih[h], q1r);
-            q1i = P::mma(nQr[h], Aih[h], q1i);
-            q0r = P::mma(Prl[h], Arh[h], q0r);
-            q0i = P::mma(Pil[h], Arh[h], q0i);
-            q0r = P::mma(Pih[h], Aih[h], q0r);
-            q0i = P::mma(nPrh[h], Aih[h], q0i);
-            q0r = P::mma(Pih[h], Ail[h], q0r);
-            q0i = P::mma(nPrh[h], Ail[h], q0i);
-            q0r = P::mma(Pil[h], Aih[h], q0r);
-            q0i = P::mma(nPrl[h], Aih[h], q0i);
-        }
+        f32x16 q0r, q0i, q1r, q1i;
+        tile_mma(Arh, Arl, Aih, Ail, q0r, q0i, q1r, q1i);
         // vis = sum_y conj(B_v[y]) (Q0 - 2 pi i c Q1) over this lane's 16 rows
         const float cf = (float)cc * q1_back;
         float sr = 0.f, si = 0.f;

@@ -81,6 +81,10 @@ struct KernelArgs {
     double w_offset_scale; // 2 pi * WStepLambda (0 in the reference)
     float degrid_scale;    // 1.0f / (S * S)  (Gridder.cs:151)
     int32_t mode;
+    // nonzero: channels are uniformly spaced in wavenumber and C % 8 == 0, so
+    // each aligned run of 8 visibilities is one timestep's 8 consecutive
+    // channels and the MFMA kernels step their phasors channel to channel
+    int32_t chan_rec;
     const WorkItem* items; // work queue (longest first)
     int32_t nitems;
     int32_t* counter;      // next item; zeroed by the launcher before each launch

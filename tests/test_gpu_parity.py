@@ -125,6 +125,40 @@ def test_subgrid_sizes_and_wstack(O, S, K, L, wst, n_ant, T, C, kernel):
     assert e_g <= TOL and e_d <= TOL
 
 
+REC_CASES = [
+    # channels, frequency jitter (Hz), work-item size: C % 8 == 0 with even
+    # spacing steps the phasors channel to channel, re-anchored every 8
+    # channels and at every item start (C = 24 items start mid-timestep);
+    # uneven spacing takes the direct evaluation
+    (16, 0.0, 160),
+    (24, 0.0, 96),
+    (8, 3.0e5, 64),
+]
+
+
+@pytest.mark.parametrize("C,jitter,chunk", REC_CASES)
+@pytest.mark.parametrize("kernel", ["mfma_bf16", "mfma_fp16"])
+def test_channel_stepping_cases(O, C, jitter, chunk, kernel):
+    ob = obs.custom(G=256, S=32, K=8, L=4, n_ant=10, T=40, C=C, f0=1.0e9, f1=1.05e9, seed=70 + C,
+                    core_diam=1200.0, noise=0.01)
+    if jitter:
+        ob.freq = ob.freq + jitter * np.sin(np.arange(C) * 1.7)
+        ob._partition = None
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    m = part.subgrids
+    ref_l = O.gridw(oc, m, ob.vis, ob.uvw, ob.freq)
+    model = ref_l.sum(axis=0)
+    ref_v = O.degridw(oc, m, model, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.set_launch(True, chunk, chunk, 0)
+        e_g = rel_err(plan.gridw(ob.vis), ref_l)
+        e_d = rel_err(plan.degridw(model), ref_v)
+    print(kernel, C, jitter, chunk, "gridw %.2e degridw %.2e" % (e_g, e_d))
+    assert e_g <= TOL and e_d <= TOL
+
+
 def test_invalid_grid_error_and_wstack_skip(O):
     ob = obs.custom(G=128, S=16, K=4, L=1, n_ant=6, T=10, C=3, seed=5)
     part = ob.partition()
