@@ -37,6 +37,7 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 // An MFMA operand of 8 16-bit values, two per dword (element 2q in the low
 // half of dword q), so pairs are converted with one v_cvt_pk instruction.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -47,6 +48,9 @@ struct PrecBF16 {
     static constexpr bool kNormalise = false;
     __device__ static __forceinline__ f32x16 mma(u32x4 a, u32x4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b), c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ f32x4 mma16(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b), c, 0, 0, 0);
     }
     __device__ static __forceinline__ uint32_t pack(float a, float b) {
         return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
@@ -61,6 +65,9 @@ struct PrecF16 {
     static constexpr bool kNormalise = true;
     __device__ static __forceinline__ f32x16 mma(u32x4 a, u32x4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b), c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ f32x4 mma16(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b), c, 0, 0, 0);
     }
     __device__ static __forceinline__ uint32_t pack(float a, float b) {
         return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, f16x2));
@@ -304,19 +311,25 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 }
 
 // ---------------------------------------------------------------------------
-// Degridder.  One workgroup per subgrid: splitter + forward FFT into LDS, the
-// subgrid pixels split into registers as the MFMA A operand (rows y, k = x),
-// then each wave takes tiles of 32 visibilities (the N dimension).
+// Degridder for S <= 32 on 16x16x32 MFMA tiles.  One workgroup per work item:
+// splitter + forward FFT into LDS, then the subgrid pixels, split hi/lo, stay
+// in registers as the MFMA A operand (row tiles of 16 y, k = x padded to 32).
+// A tile is 16 visibilities (the N dimension): lane l holds the x-phasors
+// of visibility l % 16 at x = 8 (l / 16) + e and accumulates output rows
+// y = 16 rt + 4 (l / 16) + r, so per lane the state is 8 x-phasors and
+// 4 RT y-phasors, small enough for two waves per SIMD.
 template <int S, class P>
 __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
-    static_assert(S <= 32, "one 32x32 MFMA tile per subgrid");
+    static_assert(S <= 32, "x fits one 32-deep k step");
+    constexpr int RT = (S + 15) / 16;  // row tiles
+    constexpr int NB = 4 * RT;         // y-phasors per lane
     __shared__ float2 s_tile[2][S * S];
     __shared__ double s_l[32], s_n1[32];
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 31, grp = lane >> 5;
+    const int col = lane & 15, grp = lane >> 4;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
 
@@ -339,17 +352,17 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     const float rscale = 1.f / pow2_ceil((float)fabs(r_corner));
 
     // A operand: pixels_copy[y][x] = subgrid[(y+S/2)%S][(x+S/2)%S] (Gridder.cs:101-117)
-    // lane holds row y = li, k = x = 16 h + 8 grp + e
-    u32x4 Prh[2], Prl[2], Pih[2], Pil[2], Qr[2], Qi[2];
+    // at y = 16 rt + col, x = 8 grp + e; P split hi/lo and r P
+    u32x4 Prh[RT], Prl[RT], Pih[RT], Pil[RT], Qr[RT], Qi[RT];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
             float2 pv[2];
             float rr[2];
 #pragma unroll
             for (int u = 0; u < 2; u++) {
-                const int y = li, x = 16 * h + 8 * grp + 2 * q + u;
+                const int y = 16 * rt + col, x = 8 * grp + 2 * q + u;
                 pv[u] = make_float2(0.f, 0.f);
                 rr[u] = 0.f;
                 if (y < S && x < S) {
@@ -359,62 +372,99 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                     rr[u] = (float)residual_n(x, y, S, a.image_size) * rscale;
                 }
             }
-            P9_SPLIT2(P, pv[0].x, pv[1].x, Prh[h][q], Prl[h][q]);
-            P9_SPLIT2(P, pv[0].y, pv[1].y, Pih[h][q], Pil[h][q]);
-            Qr[h][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
-            Qi[h][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
+            P9_SPLIT2(P, pv[0].x, pv[1].x, Prh[rt][q], Prl[rt][q]);
+            P9_SPLIT2(P, pv[0].y, pv[1].y, Pih[rt][q], Pil[rt][q]);
+            Qr[rt][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
+            Qi[rt][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
     }
     const float out_scale = a.degrid_scale / pscale;
     const float q1_back = kTwoPiF / rscale;
 
-    // Q0 = P conj(A), Q1 = (r P) conj(A) for one tile of 32 visibilities
-    auto tile_mma = [&](const u32x4 (&Arh)[2], const u32x4 (&Arl)[2], const u32x4 (&Aih)[2],
-                        const u32x4 (&Ail)[2], f32x16& q0r, f32x16& q0i, f32x16& q1r, f32x16& q1i) {
-        q0r = f32x16{};
-        q0i = f32x16{};
-        q1r = f32x16{};
-        q1i = f32x16{};
+    // One tile: Q0 = P conj(A), Q1 = (r P) conj(A) on the MFMA, then
+    // vis = sum_y conj(B_v[y]) (Q0 - 2 pi i c Q1) over this lane's rows and
+    // the four lane groups.  A: the 8 x-phasors, B: the NB y-phasors.
+    auto tile = [&](const float2 (&A)[8], const float2 (&B)[NB], float cf) -> float2 {
+        u32x4 Arh, Arl, Aih, Ail;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const u32x4 nAih = neg(Aih[h]), nAil = neg(Ail[h]);
-            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai)
-            q0r = P::mma(Prh[h], Arh[h], q0r);
-            q0i = P::mma(Pih[h], Arh[h], q0i);
-            q1r = P::mma(Qr[h], Arh[h], q1r);
-            q1i = P::mma(Qi[h], Arh[h], q1i);
-            q0r = P::mma(Prh[h], Arl[h], q0r);
-            q0i = P::mma(Pih[h], Arl[h], q0i);
-            q1r = P::mma(Qi[h], Aih[h], q1r);
-            q1i = P::mma(Qr[h], nAih, q1i);
-            q0r = P::mma(Prl[h], Arh[h], q0r);
-            q0i = P::mma(Pil[h], Arh[h], q0i);
-            q0r = P::mma(Pih[h], Aih[h], q0r);
-            q0i = P::mma(Prh[h], nAih, q0i);
-            q0r = P::mma(Pih[h], Ail[h], q0r);
-            q0i = P::mma(Prh[h], nAil, q0i);
-            q0r = P::mma(Pil[h], Aih[h], q0r);
-            q0i = P::mma(Prl[h], nAih, q0i);
+        for (int q = 0; q < 4; q++) {
+            P9_SPLIT2(P, A[2 * q].x, A[2 * q + 1].x, Arh[q], Arl[q]);
+            P9_SPLIT2(P, A[2 * q].y, A[2 * q + 1].y, Aih[q], Ail[q]);
+        }
+        const u32x4 nAih = neg(Aih), nAil = neg(Ail);
+        f32x4 q0r[RT], q0i[RT], q1r[RT], q1i[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai) (hi*hi + hi*lo + lo*hi)
+            f32x4 z = {};
+            q0r[rt] = P::mma16(Prh[rt], Arh, z);
+            q0i[rt] = P::mma16(Pih[rt], Arh, z);
+            q1r[rt] = P::mma16(Qr[rt], Arh, z);
+            q1i[rt] = P::mma16(Qi[rt], Arh, z);
+            q0r[rt] = P::mma16(Prh[rt], Arl, q0r[rt]);
+            q0i[rt] = P::mma16(Pih[rt], Arl, q0i[rt]);
+            q1r[rt] = P::mma16(Qi[rt], Aih, q1r[rt]);
+            q1i[rt] = P::mma16(Qr[rt], nAih, q1i[rt]);
+            q0r[rt] = P::mma16(Prl[rt], Arh, q0r[rt]);
+            q0i[rt] = P::mma16(Pil[rt], Arh, q0i[rt]);
+            q0r[rt] = P::mma16(Pih[rt], Aih, q0r[rt]);
+            q0i[rt] = P::mma16(Prh[rt], nAih, q0i[rt]);
+            q0r[rt] = P::mma16(Pih[rt], Ail, q0r[rt]);
+            q0i[rt] = P::mma16(Prh[rt], nAil, q0i[rt]);
+            q0r[rt] = P::mma16(Pil[rt], Aih, q0r[rt]);
+            q0i[rt] = P::mma16(Prl[rt], nAih, q0i[rt]);
+        }
+        float sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float2 b = B[4 * rt + r];
+                const float zr = q0r[rt][r] + cf * q1i[rt][r];
+                const float zi = q0i[rt][r] - cf * q1r[rt][r];
+                sr = fmaf(b.x, zr, fmaf(b.y, zi, sr));
+                si = fmaf(b.x, zi, fmaf(-b.y, zr, si));
+            }
+        }
+        sr += __shfl_xor(sr, 16);
+        si += __shfl_xor(si, 16);
+        sr += __shfl_xor(sr, 32);
+        si += __shfl_xor(si, 32);
+        return make_float2(sr * out_scale, si * out_scale);
+    };
+    // x of phasor e and y of phasor i for this lane; phasors outside the
+    // subgrid meet zero pixels and are kept at 0 (finite)
+    auto xphasors = [&](double ca, double cc, float2 (&A)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int x = 8 * grp + e;
+            A[e] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
+        }
+    };
+    auto yphasors = [&](double cb, double cc, float2 (&B)[NB]) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int y = 16 * (i / 4) + 4 * grp + (i % 4);
+            B[i] = y < S ? cis_f64(fma(cb, s_l[y], cc * s_n1[y])) : make_float2(0.f, 0.f);
         }
     };
 
     if (a.chan_rec) {
-        // Channel recurrence.  Column li of every tile is one timestep and
+        // Channel recurrence.  Column col of every tile is one timestep and
         // consecutive tiles are consecutive channels, so each lane steps its
-        // 16 x-phasors and 16 y-phasors by one multiplication per tile.  A
-        // super-tile is 32 timesteps x C channels; the (super-tile, channel)
-        // sequence is cut into 4 equal contiguous parts, one per wave, and the
-        // phasors are evaluated directly at the start of every run and every
-        // 8 channels.
+        // phasors by one multiplication per tile.  A super-tile is 16
+        // timesteps x C channels; the (super-tile, channel) sequence is cut
+        // into 4 equal contiguous parts, one per wave, and the phasors are
+        // evaluated directly at the start of every run and every 8 channels.
         const int tlo = k0 / C, thi = (K + C - 1) / C;
-        const int nst = (thi - tlo + 31) / 32;
+        const int nst = (thi - tlo + 15) / 16;
         const int64_t F = (int64_t)nst * C;
         int64_t f = F * wave / 4;
         const int64_t f_end = F * (wave + 1) / 4;
         while (f < f_end) {
             const int st = (int)(f / C), ch0 = (int)(f % C);
             const int run = (int)min((int64_t)(C - ch0), f_end - f);
-            const int tau = tlo + st * 32 + li;  // this lane's timestep in the subgrid
+            const int tau = tlo + st * 16 + col;  // this lane's timestep in the subgrid
             const double* p = a.uvw + ((size_t)g.bl * a.T + g.t0 + min(tau, g.nt - 1)) * 3;
             const double u = p[0], v = p[1], w = p[2];
 #pragma unroll 1
@@ -422,54 +472,23 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 const int ch = ch0 + cb;
                 const double k = a.wavenumbers[ch];
                 const double dk = a.wavenumbers[ch + 1 < C ? ch + 1 : ch] - k;
-                const double ca = (g.u_off - k * u) * kInvTwoPi, cbv = (g.v_off - k * v) * kInvTwoPi;
-                const double cc = (g.w_off - k * w) * kInvTwoPi;
-                const double da = -dk * u * kInvTwoPi, db = -dk * v * kInvTwoPi, dc = -dk * w * kInvTwoPi;
-                float2 As[16], RA[16], Bs[16], RB[16];
-#pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    const int x = 16 * (i / 8) + 8 * grp + (i % 8);
-                    const int y = 8 * (i / 4) + 4 * grp + (i % 4);
-                    // padded columns (x >= S) meet zero pixels: keep them finite
-                    As[i] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
-                    RA[i] = x < S ? cis_f64(fma(da, s_l[x], dc * s_n1[x])) : make_float2(0.f, 0.f);
-                    Bs[i] = y < S ? cis_f64(fma(cbv, s_l[y], cc * s_n1[y])) : make_float2(0.f, 0.f);
-                    RB[i] = y < S ? cis_f64(fma(db, s_l[y], dc * s_n1[y])) : make_float2(0.f, 0.f);
-                }
+                const double cc = (g.w_off - k * w) * kInvTwoPi, dc = -dk * w * kInvTwoPi;
+                float2 A[8], RA[8], B[NB], RB[NB];
+                xphasors((g.u_off - k * u) * kInvTwoPi, cc, A);
+                xphasors(-dk * u * kInvTwoPi, dc, RA);
+                yphasors((g.v_off - k * v) * kInvTwoPi, cc, B);
+                yphasors(-dk * v * kInvTwoPi, dc, RB);
                 const float cf0 = (float)cc * q1_back, dcf = (float)dc * q1_back;
                 const int ne = min(8, run - cb);
 #pragma unroll 1
                 for (int e = 0; e < ne; e++) {
-                    u32x4 Arh[2], Arl[2], Aih[2], Ail[2];
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-#pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            P9_SPLIT2(P, As[8 * h + 2 * q].x, As[8 * h + 2 * q + 1].x, Arh[h][q], Arl[h][q]);
-                            P9_SPLIT2(P, As[8 * h + 2 * q].y, As[8 * h + 2 * q + 1].y, Aih[h][q], Ail[h][q]);
-                        }
-                    }
-                    f32x16 q0r, q0i, q1r, q1i;
-                    tile_mma(Arh, Arl, Aih, Ail, q0r, q0i, q1r, q1i);
-                    const float cf = fmaf((float)e, dcf, cf0);
-                    float sr = 0.f, si = 0.f;
-#pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const float zr = q0r[r] + cf * q1i[r];
-                        const float zi = q0i[r] - cf * q1r[r];
-                        sr = fmaf(Bs[r].x, zr, fmaf(Bs[r].y, zi, sr));
-                        si = fmaf(Bs[r].x, zi, fmaf(-Bs[r].y, zr, si));
-                    }
-                    sr += __shfl_xor(sr, 32);
-                    si += __shfl_xor(si, 32);
+                    const float2 vis = tile(A, B, fmaf((float)e, dcf, cf0));
                     const int j = tau * C + ch + e;
-                    if (grp == 0 && tau < thi && j >= k0 && j < K)
-                        a.vis_out[vbase + j] = make_float2(sr * out_scale, si * out_scale);
+                    if (grp == 0 && tau < thi && j >= k0 && j < K) a.vis_out[vbase + j] = vis;
 #pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        As[i] = cmulf(As[i], RA[i]);
-                        Bs[i] = cmulf(Bs[i], RB[i]);
-                    }
+                    for (int i = 0; i < 8; i++) A[i] = cmulf(A[i], RA[i]);
+#pragma unroll
+                    for (int i = 0; i < NB; i++) B[i] = cmulf(B[i], RB[i]);
                 }
             }
             f += run;
@@ -477,53 +496,20 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
         return;
     }
 
-    const int ntiles = (K - k0 + 31) / 32;
+    // direct evaluation: tiles of 16 consecutive visibilities, round robin
+    const int ntiles = (K - k0 + 15) / 16;
     for (int vt = wave; vt < ntiles; vt += 4) {
-        const int j = k0 + vt * 32 + li;
+        const int j = k0 + vt * 16 + col;
         const int jc = j < K ? j : K - 1;
         const int t = g.t0 + jc / C, ch = jc % C;
         const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
         const double k = a.wavenumbers[ch];
-        const double ca = (g.u_off - k * p[0]) * kInvTwoPi;
-        const double cb = (g.v_off - k * p[1]) * kInvTwoPi;
         const double cc = (g.w_off - k * p[2]) * kInvTwoPi;
-
-        // B operand: A_v[x] for this lane's vis (column) and x = 16 h + 8 grp + e
-        u32x4 Arh[2], Arl[2], Aih[2], Ail[2];
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                float2 A[2];
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const int x = 16 * h + 8 * grp + 2 * q + u;
-                    // padded columns (x >= S) meet zero pixels: keep them finite
-                    A[u] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
-                }
-                P9_SPLIT2(P, A[0].x, A[1].x, Arh[h][q], Arl[h][q]);
-                P9_SPLIT2(P, A[0].y, A[1].y, Aih[h][q], Ail[h][q]);
-            }
-        }
-        f32x16 q0r, q0i, q1r, q1i;
-        tile_mma(Arh, Arl, Aih, Ail, q0r, q0i, q1r, q1i);
-        // vis = sum_y conj(B_v[y]) (Q0 - 2 pi i c Q1) over this lane's 16 rows
-        const float cf = (float)cc * q1_back;
-        float sr = 0.f, si = 0.f;
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int y = 8 * (r / 4) + 4 * grp + (r % 4);
-            if (y < S) {
-                const float2 B = cis_f64(fma(cb, s_l[y], cc * s_n1[y]));
-                const float zr = q0r[r] + cf * q1i[r];
-                const float zi = q0i[r] - cf * q1r[r];
-                sr = fmaf(B.x, zr, fmaf(B.y, zi, sr));
-                si = fmaf(B.x, zi, fmaf(-B.y, zr, si));
-            }
-        }
-        sr += __shfl_xor(sr, 32);
-        si += __shfl_xor(si, 32);
-        if (grp == 0 && j < K) a.vis_out[vbase + j] = make_float2(sr * out_scale, si * out_scale);
+        float2 A[8], B[NB];
+        xphasors((g.u_off - k * p[0]) * kInvTwoPi, cc, A);
+        yphasors((g.v_off - k * p[1]) * kInvTwoPi, cc, B);
+        const float2 vis = tile(A, B, (float)cc * q1_back);
+        if (grp == 0 && j < K) a.vis_out[vbase + j] = vis;
     }
 }
 
@@ -686,7 +672,9 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_queue_kernel(KernelArgs a)
     run_queue(a, [&](int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
 }
 
-// S <= 32: one tile; S = 48, 64: the wide form
+// S <= 32: 16x16x32 tiles, two waves per SIMD; S = 48, 64: the wide form
+template <int S>
+constexpr int kDegridWaves = S <= 32 ? 2 : 1;
 template <int S, class P>
 __device__ __forceinline__ void degrid_mfma_any(const KernelArgs& a, int s, int k0, int k1) {
     if constexpr (S <= 32)
@@ -695,11 +683,11 @@ __device__ __forceinline__ void degrid_mfma_any(const KernelArgs& a, int s, int 
         degrid_mfma_wide_body<S, P>(a, s, k0, k1);
 }
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void degrid_mfma_kernel(KernelArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_kernel(KernelArgs a) {
     degrid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1);
 }
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void degrid_mfma_queue_kernel(KernelArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_queue_kernel(KernelArgs a) {
     run_queue(a, [&](int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
 }
 
@@ -708,7 +696,7 @@ __global__ __launch_bounds__(kThreads) void grid_mfma_batch_kernel(BatchArgs b) 
     run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
 }
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void degrid_mfma_batch_kernel(BatchArgs b) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_batch_kernel(BatchArgs b) {
     run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
 }
 
