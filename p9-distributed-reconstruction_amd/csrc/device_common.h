@@ -108,15 +108,18 @@ __device__ __forceinline__ void dft_small(float2* v, int sign) {
 }
 
 // One Stockham pass of radix R over all S lines of the tile.  Line b's
-// element i lives at b*line_stride + i*elem_stride.
+// element i lives at b*line_stride + i*elem_stride.  When the lines are the
+// columns (line_stride 1), consecutive threads take consecutive lines so a
+// wave's LDS accesses are contiguous instead of S elements apart.
 template <int S, int R>
 __device__ __forceinline__ void fft_pass(const float2* __restrict__ in, float2* __restrict__ out,
                                          const float2* tw, int Ns, int sign, int line_stride,
                                          int elem_stride) {
     constexpr int M = S / R;
     for (int idx = threadIdx.x; idx < S * M; idx += kThreads) {
-        const int b = idx / M;
-        const int j = idx - b * M;
+        const bool cols = line_stride == 1;
+        const int b = cols ? idx % S : idx / M;
+        const int j = cols ? idx / S : idx - b * M;
         const int k = j % Ns;
         float2 v[R];
 #pragma unroll

@@ -146,7 +146,8 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
     constexpr int NT = (S + 31) / 32;  // tiles per axis
     constexpr int WPK = 4 / (NT * NT); // waves sharing one tile's visibilities
     static_assert(NT * 32 >= S && NT <= 2, "subgrids up to 64");
-    constexpr int ROUND = kThreads;
+    constexpr int PER = 2;  // visibilities staged per thread and round
+    constexpr int ROUND = PER * kThreads;
     __shared__ double s_a[ROUND], s_b[ROUND], s_c[ROUND];
     __shared__ float2 s_v[ROUND];
     __shared__ float2 s_tile[2][S * S];
@@ -185,26 +186,37 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 
     f32x16 acc0r = {}, acc0i = {}, acc1r = {}, acc1i = {};
 
+    // The next round's uvw, wavenumbers and visibilities are fetched into
+    // registers while the current round is on the MFMA, then staged in LDS
+    // as FP64 phase coefficients (a, b, c) and scaled visibilities.
+    double fu[PER], fv[PER], fw[PER], fk[PER];
+    float2 fvis[PER];
+    auto fetch = [&](int j0) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int jc = min(j0 + i * kThreads + tid, K - 1);
+            const int t = g.t0 + jc / C, ch = jc % C;
+            const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
+            fu[i] = p[0];
+            fv[i] = p[1];
+            fw[i] = p[2];
+            fk[i] = a.wavenumbers[ch];
+            fvis[i] = a.vis[vbase + jc];
+        }
+    };
+    if (k0 < K) fetch(k0);
     for (int j0 = k0; j0 < K; j0 += ROUND) {
-        {
-            const int j = j0 + tid;
-            if (j < K) {
-                const int t = g.t0 + j / C, ch = j % C;
-                const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
-                const double k = a.wavenumbers[ch];
-                s_a[tid] = (g.u_off - k * p[0]) * kInvTwoPi;
-                s_b[tid] = (g.v_off - k * p[1]) * kInvTwoPi;
-                s_c[tid] = (g.w_off - k * p[2]) * kInvTwoPi;
-                const float2 v = a.vis[vbase + j];
-                s_v[tid] = make_float2(v.x * vscale, v.y * vscale);
-            } else {
-                s_a[tid] = 0.0;
-                s_b[tid] = 0.0;
-                s_c[tid] = 0.0;
-                s_v[tid] = make_float2(0.f, 0.f);
-            }
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int idx = i * kThreads + tid;
+            const bool in = j0 + idx < K;
+            s_a[idx] = in ? (g.u_off - fk[i] * fu[i]) * kInvTwoPi : 0.0;
+            s_b[idx] = in ? (g.v_off - fk[i] * fv[i]) * kInvTwoPi : 0.0;
+            s_c[idx] = in ? (g.w_off - fk[i] * fw[i]) * kInvTwoPi : 0.0;
+            s_v[idx] = in ? make_float2(fvis[i].x * vscale, fvis[i].y * vscale) : make_float2(0.f, 0.f);
         }
         __syncthreads();
+        if (j0 + ROUND < K) fetch(j0 + ROUND);
         const int nsteps = (min(ROUND, K - j0) + 15) / 16;
         for (int st = kpart; st < nsteps; st += WPK) {
             u32x4 Lrh, Lrl, Lih, Lil, cLr, cLi, Arh, Arl, Aih, Ail;
@@ -664,11 +676,11 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
 }
 
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void grid_mfma_kernel(KernelArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void grid_mfma_kernel(KernelArgs a) {
     grid_mfma_body<S, P>(a, a.order[blockIdx.x], 0, -1);
 }
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void grid_mfma_queue_kernel(KernelArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void grid_mfma_queue_kernel(KernelArgs a) {
     run_queue(a, [&](int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
 }
 
@@ -692,7 +704,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
 }
 
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) void grid_mfma_batch_kernel(BatchArgs b) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void grid_mfma_batch_kernel(BatchArgs b) {
     run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
 }
 template <int S, class P>
