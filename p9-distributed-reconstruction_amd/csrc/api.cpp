@@ -66,6 +66,7 @@ struct p9idg_plan {
     double* d_image64 = nullptr;
     float2* d_twiddle = nullptr;
     float2* d_phasor = nullptr;
+    float* d_rtab = nullptr;  // [S][S] w-term residual of the MFMA phasor factorisation
     uint8_t* d_flags = nullptr;
     void* d_staging = nullptr;
     size_t staging_bytes = 0;
@@ -96,13 +97,15 @@ struct p9idg_plan {
     int32_t grid_dirty_fp = -1, layers_dirty_fp = -1;
     Box* d_fp[3] = {nullptr, nullptr, nullptr};  // per grid mode
     int32_t n_fp[3] = {0, 0, 0};
+    Box fp_union[3] = {};                         // bounding box of each mode's boxes
+    float2* d_lsum = nullptr;                     // layer sum for the w-stacked ToImage
     int32_t use_queue = 1;
     int32_t grid_chunk = 2048;
     int32_t degrid_chunk = 2048;
     int32_t workgroups = 0;
 
-    hipfftHandle fft_one = 0, fft_layers = 0;
-    bool have_fft_one = false, have_fft_layers = false;
+    hipfftHandle fft_one = 0;
+    bool have_fft_one = false;
 
     hipEvent_t ev[8] = {};
     bool timed_grid = false, timed_degrid = false, timed_fft = false;
@@ -202,8 +205,16 @@ int build_footprints(p9idg_plan* p, const p9idg_subgrid* meta, int64_t nsub) {
             b.x1 = std::max(b.x1, X + S);
         }
         std::vector<Box> used;
+        Box u{0, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
         for (const Box& b : box)
-            if (b.y1 > b.y0) used.push_back(b);
+            if (b.y1 > b.y0) {
+                used.push_back(b);
+                u.y0 = std::min(u.y0, b.y0);
+                u.y1 = std::max(u.y1, b.y1);
+                u.x0 = std::min(u.x0, b.x0);
+                u.x1 = std::max(u.x1, b.x1);
+            }
+        p->fp_union[mode] = u;
         dev_free(p->d_fp[mode]);
         p->n_fp[mode] = (int32_t)used.size();
         if (used.empty()) continue;
@@ -234,16 +245,6 @@ int ensure_fft_one(p9idg_plan* p) {
     P9_FFT(hipfftPlan2d(&p->fft_one, p->c.grid_size, p->c.grid_size, HIPFFT_C2C));
     P9_FFT(hipfftSetStream(p->fft_one, p->stream));
     p->have_fft_one = true;
-    return P9IDG_OK;
-}
-
-int ensure_fft_layers(p9idg_plan* p) {
-    if (p->have_fft_layers) return P9IDG_OK;
-    int n[2] = {p->c.grid_size, p->c.grid_size};
-    P9_FFT(hipfftPlanMany(&p->fft_layers, 2, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_C2C,
-                          p->c.wlayer_count));
-    P9_FFT(hipfftSetStream(p->fft_layers, p->stream));
-    p->have_fft_layers = true;
     return P9IDG_OK;
 }
 
@@ -375,6 +376,7 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
     a.vis_out = p->d_model_vis;
     a.twiddle = p->d_twiddle;
     a.adder_phasor = p->d_phasor;
+    a.rtab = p->d_rtab;
     a.model_grid = p->d_model_grid;
     a.G = p->c.grid_size;
     a.T = p->T;
@@ -452,6 +454,30 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     return P9IDG_OK;
 }
 
+// FFT.WStackIFFTFloat (FFT.cs:188-215): sum over layers of
+// (float)(Re(IFFT(layer)) / N).  The inverse FFT is linear and applies no
+// per-layer w-screen, so it runs once on the sum of the layers (identical up
+// to the per-layer float rounding).  The layers are left as they are; when
+// their footprint is known only the union of the footprint boxes is summed.
+int layers_image(p9idg_plan* p, double inv_n, int shift) {
+    const int G = p->c.grid_size;
+    const int W = P9IDG_GRID_WSTACK;
+    if (int st = dev_alloc(&p->d_lsum, grid_elems(p))) return st;
+    if (int st = ensure_fft_one(p)) return st;
+    if (p->layers_clean && p->layers_dirty_fp == W && !p->capturing) {
+        P9_HIP(launch_fill_zero(p->d_lsum, grid_elems(p), p->stream));
+        if (p->n_fp[W] > 0)
+            P9_HIP(launch_sum_layer_boxes(p->d_layers, p->d_fp[W], p->n_fp[W], G, p->fp_union[W], p->d_lsum,
+                                          p->stream));
+    } else {
+        P9_HIP(launch_sum_layers(p->d_layers, p->c.wlayer_count, grid_elems(p), p->d_lsum, p->stream));
+    }
+    P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_lsum),
+                         reinterpret_cast<hipfftComplex*>(p->d_lsum), HIPFFT_BACKWARD));
+    P9_HIP(launch_layers_to_image(p->d_lsum, 1, G, inv_n, shift, p->d_image, p->stream));
+    return P9IDG_OK;
+}
+
 // Inverse FFT of the grid buffer(s) followed by the image extraction.
 int do_grid_to_image(p9idg_plan* p, int mode) {
     if (int st = check_obs(p)) return st;
@@ -461,12 +487,7 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
     if (int st_ = record(p, 5)) return st_;
     if (mode == P9IDG_GRID_WSTACK) {
         if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
-        p->layers_clean = false;
-        if (int st = ensure_fft_layers(p)) return st;
-        P9_FFT(hipfftExecC2C(p->fft_layers, reinterpret_cast<hipfftComplex*>(p->d_layers),
-                             reinterpret_cast<hipfftComplex*>(p->d_layers), HIPFFT_BACKWARD));
-        P9_HIP(launch_layers_to_image(p->d_layers, p->c.wlayer_count, G, inv_n, 1, p->d_image,
-                                      p->stream));
+        if (int st = layers_image(p, inv_n, 1)) return st;
     } else {
         p->grid_clean = false;
         if (int st = ensure_fft_one(p)) return st;
@@ -656,6 +677,10 @@ int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
     if (e != hipSuccess) return bail(hip_fail(e, "twiddle upload"));
     e = hipMemcpy(p->d_phasor, ph.data(), ph.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(hip_fail(e, "phasor upload"));
+    if ((st = dev_alloc(&p->d_rtab, (size_t)S * S))) return bail(st);
+    e = launch_rtab(p->d_rtab, S, p->c.scale_arcsec * (float)G, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return bail(hip_fail(e, "residual table"));
     *plan = p;
     return P9IDG_OK;
 }
@@ -665,7 +690,6 @@ int p9idg_destroy(p9idg_plan* p) {
     DeviceGuard dg(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->have_fft_one) (void)hipfftDestroy(p->fft_one);
-    if (p->have_fft_layers) (void)hipfftDestroy(p->fft_layers);
     dev_free(p->d_uvw);
     dev_free(p->d_wavenumbers);
     dev_free(p->d_meta);
@@ -677,8 +701,10 @@ int p9idg_destroy(p9idg_plan* p) {
     dev_free(p->d_model_grid);
     dev_free(p->d_image);
     dev_free(p->d_image64);
+    dev_free(p->d_lsum);
     dev_free(p->d_twiddle);
     dev_free(p->d_phasor);
+    dev_free(p->d_rtab);
     dev_free(p->d_flags);
     dev_free(p->d_counter);
     dev_free(p->d_resid);
@@ -928,14 +954,10 @@ int p9idg_wstack_ifft_float(p9idg_plan* p, const double* layers, int32_t L, int6
     if (vis_count == 0) return fail(P9IDG_ERR_ARGUMENT, "vis_count is 0");
     DeviceGuard dg(p->device);
     if (int st = ensure_layers(p)) return st;
-    if (int st = ensure_fft_layers(p)) return st;
     const size_t n = grid_elems(p) * L;
     p->layers_clean = false;
     if (int st = upload_c128(p, layers, p->d_layers, n)) return st;
-    P9_FFT(hipfftExecC2C(p->fft_layers, reinterpret_cast<hipfftComplex*>(p->d_layers),
-                         reinterpret_cast<hipfftComplex*>(p->d_layers), HIPFFT_BACKWARD));
-    P9_HIP(launch_layers_to_image(p->d_layers, L, p->c.grid_size, 1.0 / (double)vis_count, 0,
-                                  p->d_image, p->stream));
+    if (int st = layers_image(p, 1.0 / (double)vis_count, 0)) return st;
     P9_HIP(hipMemcpyAsync(out, p->d_image, grid_elems(p) * sizeof(float), hipMemcpyDeviceToHost,
                           p->stream));
     return sync(p);

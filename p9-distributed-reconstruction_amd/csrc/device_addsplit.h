@@ -18,7 +18,7 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
     const int G = a.G;
     if (a.mode == P9IDG_GRID_PLAIN) {
         // Adder.cs:22-47 (range validated on the host: 0 <= U,V < G-S)
-        for (int p = threadIdx.x; p < S * S; p += kThreads) {
+        for (int p = item_tid(); p < S * S; p += kThreads) {
             const int y = p / S, x = p - (p / S) * S;
             const int ys = (y + S / 2) % S, xs = (x + S / 2) % S;
             float2 v = cmulf(tile[ys * S + xs], a.adder_phasor[p]);
@@ -41,7 +41,7 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
     if (!(X >= 1 && X < G - S && Y >= 1 && Y < G - S)) return;
     if (W < 0 || W >= a.L) return;  // metadata validated on the host
     float2* layer = a.grid + (a.mode == P9IDG_GRID_WSUM ? 0 : (size_t)W * G * G);
-    for (int p = threadIdx.x; p < S * S; p += kThreads) {
+    for (int p = item_tid(); p < S * S; p += kThreads) {
         const int y = p / S, x = p - (p / S) * S;
         const int y_ = neg ? S - 1 - y : y;
         const int x_ = neg ? S - 1 - x : x;
@@ -65,7 +65,7 @@ __device__ __forceinline__ void splitter(const KernelArgs& a, const SubgridGeom&
     if (a.mode == P9IDG_DEGRID_PLAIN) {
         // Adder.cs:65-90
         const bool ok = g.U >= 0 && g.U < G - S && g.V >= 0 && g.V < G - S;
-        for (int p = threadIdx.x; p < S * S; p += kThreads) {
+        for (int p = item_tid(); p < S * S; p += kThreads) {
             const int y = p / S, x = p - (p / S) * S;
             const int yd = (y + S / 2) % S, xd = (x + S / 2) % S;
             float2 v = make_float2(0.f, 0.f);
@@ -82,7 +82,7 @@ __device__ __forceinline__ void splitter(const KernelArgs& a, const SubgridGeom&
     const bool neg = g.W < 0;
     const int X = g.U, Y = g.V;
     const bool ok = X >= 1 && X < G - S && Y >= 1 && Y < G - S;
-    for (int p = threadIdx.x; p < S * S; p += kThreads) {
+    for (int p = item_tid(); p < S * S; p += kThreads) {
         const int y = p / S, x = p - (p / S) * S;
         const int yd = (y + S / 2) % S, xd = (x + S / 2) % S;
         float2 v = make_float2(0.f, 0.f);

@@ -13,6 +13,15 @@ namespace dev {
 constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
+// threadIdx.x behind an empty volatile asm: values derived from it are
+// recomputed where they are used instead of being hoisted out of the
+// persistent work-item loop and held (or spilled) across items.
+__device__ __forceinline__ int item_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -116,7 +125,7 @@ __device__ __forceinline__ void fft_pass(const float2* __restrict__ in, float2* 
                                          const float2* tw, int Ns, int sign, int line_stride,
                                          int elem_stride) {
     constexpr int M = S / R;
-    for (int idx = threadIdx.x; idx < S * M; idx += kThreads) {
+    for (int idx = item_tid(); idx < S * M; idx += kThreads) {
         const bool cols = line_stride == 1;
         const int b = cols ? idx % S : idx / M;
         const int j = cols ? idx / S : idx - b * M;

@@ -116,6 +116,12 @@ __device__ __forceinline__ double residual_n(int x, int y, int S, float image_si
     return (double)compute_n(l, m) - (double)compute_n(l, 0.f) - (double)compute_n(0.f, m);
 }
 
+// r(x, y) for every pixel of the subgrid, [y][x]; built once per plan.
+__global__ void rtab_kernel(float* out, int S, float image_size) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S * S) out[i] = (float)residual_n(i % S, i / S, S, image_size);
+}
+
 // Power of two >= |x| (1 for x == 0); exact normaliser.
 __device__ __forceinline__ float pow2_ceil(float x) {
     if (!(x > 0.f)) return 1.f;
@@ -154,7 +160,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
@@ -306,7 +312,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
             for (int r = 0; r < 16; r++) {
                 const int y = ty * 32 + 8 * (r / 4) + 4 * grp + (r % 4), x = tx * 32 + li;
                 if (y < S && x < S) {
-                    const float rr = (float)(kTwoPi * residual_n(x, y, S, a.image_size));
+                    const float rr = kTwoPiF * a.rtab[y * S + x];
                     const float m1r = acc1r[r] * back1, m1i = acc1i[r] * back1;
                     const float gr = acc0r[r] * back - rr * m1i;
                     const float gi = acc0i[r] * back + rr * m1r;
@@ -340,7 +346,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, grp = lane >> 4;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
@@ -360,8 +366,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
         for (int p = tid; p < S * S; p += kThreads) m = fmaxf(m, fmaxf(fabsf(res[p].x), fabsf(res[p].y)));
         pscale = 1.f / pow2_ceil(block_max(m, s_red));
     }
-    const double r_corner = residual_n(0, 0, S, a.image_size);
-    const float rscale = 1.f / pow2_ceil((float)fabs(r_corner));
+    const float rscale = 1.f / pow2_ceil(fabsf(a.rtab[0]));
 
     // A operand: pixels_copy[y][x] = subgrid[(y+S/2)%S][(x+S/2)%S] (Gridder.cs:101-117)
     // at y = 16 rt + col, x = 8 grp + e; P split hi/lo and r P
@@ -381,7 +386,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                     pv[u] = res[((y + S / 2) % S) * S + (x + S / 2) % S];
                     pv[u].x *= pscale;
                     pv[u].y *= pscale;
-                    rr[u] = (float)residual_n(x, y, S, a.image_size) * rscale;
+                    rr[u] = a.rtab[y * S + x] * rscale;
                 }
             }
             P9_SPLIT2(P, pv[0].x, pv[1].x, Prh[rt][q], Prl[rt][q]);
@@ -492,16 +497,23 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 yphasors(-dk * v * kInvTwoPi, dc, RB);
                 const float cf0 = (float)cc * q1_back, dcf = (float)dc * q1_back;
                 const int ne = min(8, run - cb);
+                // every lane holds the sums; lane group grp keeps channels
+                // ch + 2 grp and ch + 2 grp + 1, so a timestep's channels are
+                // written by two back-to-back stores of whole cache lines
+                float2 keep0 = make_float2(0.f, 0.f), keep1 = keep0;
 #pragma unroll 1
                 for (int e = 0; e < ne; e++) {
                     const float2 vis = tile(A, B, fmaf((float)e, dcf, cf0));
-                    const int j = tau * C + ch + e;
-                    if (grp == 0 && tau < thi && j >= k0 && j < K) a.vis_out[vbase + j] = vis;
+                    if (e == 2 * grp) keep0 = vis;
+                    if (e == 2 * grp + 1) keep1 = vis;
 #pragma unroll
                     for (int i = 0; i < 8; i++) A[i] = cmulf(A[i], RA[i]);
 #pragma unroll
                     for (int i = 0; i < NB; i++) B[i] = cmulf(B[i], RB[i]);
                 }
+                const int j0 = tau * C + ch + 2 * grp;
+                if (2 * grp < ne && tau < thi && j0 >= k0 && j0 < K) a.vis_out[vbase + j0] = keep0;
+                if (2 * grp + 1 < ne && tau < thi && j0 + 1 >= k0 && j0 + 1 < K) a.vis_out[vbase + j0 + 1] = keep1;
             }
             f += run;
         }
@@ -540,7 +552,7 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
@@ -559,8 +571,7 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
         for (int p = tid; p < S * S; p += kThreads) m = fmaxf(m, fmaxf(fabsf(res[p].x), fabsf(res[p].y)));
         pscale = 1.f / pow2_ceil(block_max(m, s_red));
     }
-    const double r_corner = residual_n(0, 0, S, a.image_size);
-    const float rscale = 1.f / pow2_ceil((float)fabs(r_corner));
+    const float rscale = 1.f / pow2_ceil(fabsf(a.rtab[0]));
 
     // operand (row tile rt, k-step h) for lane ln: row y = 32 rt + ln % 32,
     // x = 16 h + 8 (ln / 32) + e; arrays Pr hi/lo, Pi hi/lo, r P re/im
@@ -581,7 +592,7 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
                     pv[u] = res[((y + S / 2) % S) * S + (x + S / 2) % S];
                     pv[u].x *= pscale;
                     pv[u].y *= pscale;
-                    rr[u] = (float)residual_n(x, y, S, a.image_size) * rscale;
+                    rr[u] = a.rtab[y * S + x] * rscale;
                 }
             }
             P9_SPLIT2(P, pv[0].x, pv[1].x, prh[w], prl[w]);
@@ -740,6 +751,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
         case 64: if (f16) P9_LAUNCH_MFMA(KERNEL, QKERNEL, 64, dev::PrecF16); else P9_LAUNCH_MFMA(KERNEL, QKERNEL, 64, dev::PrecBF16); break; \
         default: return hipErrorNotSupported;                                        \
     }
+
+hipError_t launch_rtab(float* rtab, int S, float image_size, hipStream_t st) {
+    hipLaunchKernelGGL(dev::rtab_kernel, dim3((S * S + 255) / 256), dim3(256), 0, st, rtab, S, image_size);
+    return hipGetLastError();
+}
 
 hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
                             hipStream_t st) {

@@ -48,6 +48,47 @@ __global__ void layers_to_image_kernel(const float2* layers, int L, int G, doubl
     }
 }
 
+// Elementwise sum of the L w-layers (any common layout).
+__global__ void sum_layers_kernel(const float2* layers, int L, size_t GG, float2* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < GG; i += stride) {
+        float2 acc = layers[i];
+        for (int k = 1; k < L; k++) {
+            const float2 v = layers[k * GG + i];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        out[i] = acc;
+    }
+}
+
+// Sum of the w-layers over the union box u of their footprints, where layer
+// b.layer is nonzero only inside box b (FFT.Shift-ed storage); `out` is zero
+// outside u.
+__global__ void sum_layer_boxes_kernel(const float2* layers, const Box* boxes, int nboxes, int G,
+                                       Box u, float2* out) {
+    const int w = u.x1 - u.x0;
+    const size_t n = (size_t)(u.y1 - u.y0) * (size_t)w;
+    const size_t GG = (size_t)G * G;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const int y = u.y0 + (int)(i / w), x = u.x0 + (int)(i % w);
+        const size_t idx = (size_t)shifted(y, G) * G + shifted(x, G);
+        float2 acc = make_float2(0.f, 0.f);
+        for (int b = 0; b < nboxes; b++) {
+            const Box bx = boxes[b];
+            if (y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1) {
+                const float2 v = layers[bx.layer * GG + idx];
+                acc.x += v.x;
+                acc.y += v.y;
+            }
+        }
+        out[idx] = acc;
+    }
+}
+
 // FFT.Backward(Complex[,], long) (FFT.cs:159-185) + Shift, double output.
 __global__ void grid_to_image_f64_kernel(const float2* grid, int G, double inv_n, int shift,
                                          double* img) {
@@ -151,6 +192,18 @@ hipError_t launch_layers_to_image(const float2* layers, int L, int G, double inv
                                   float* img, hipStream_t st) {
     hipLaunchKernelGGL(dev::layers_to_image_kernel, dim3(dev::blocks_for((size_t)G * G)),
                        dim3(256), 0, st, layers, L, G, inv_n, shift, img);
+    return hipGetLastError();
+}
+hipError_t launch_sum_layers(const float2* layers, int L, size_t GG, float2* out, hipStream_t st) {
+    hipLaunchKernelGGL(dev::sum_layers_kernel, dim3(dev::blocks_for(GG)), dim3(256), 0, st, layers, L,
+                       GG, out);
+    return hipGetLastError();
+}
+hipError_t launch_sum_layer_boxes(const float2* layers, const Box* boxes, int nboxes, int G, Box u,
+                                  float2* out, hipStream_t st) {
+    const size_t n = (size_t)(u.y1 - u.y0) * (size_t)(u.x1 - u.x0);
+    hipLaunchKernelGGL(dev::sum_layer_boxes_kernel, dim3(dev::blocks_for(n)), dim3(256), 0, st,
+                       layers, boxes, nboxes, G, u, out);
     return hipGetLastError();
 }
 hipError_t launch_grid_to_image_f64(const float2* grid, int G, double inv_n, int shift,

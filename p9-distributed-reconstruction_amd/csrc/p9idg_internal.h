@@ -73,6 +73,7 @@ struct KernelArgs {
     float2* vis_out;       // [B][T][C] (degrid)
     const float2* twiddle; // [S] exp(+2 pi i m / S)
     const float2* adder_phasor;  // [S][S] exp(i pi (x + y - S) / S)
+    const float* rtab;     // [S][S] r(x, y) = n(l, m) - n(l, 0) - n(0, m) (MFMA kernels)
     float2* grid;          // grid or layer stack (gridder output)
     const float2* model_grid;  // [G][G] (degrid input, unshifted as the caller passes it)
     int32_t G, T, C, L;
@@ -106,6 +107,7 @@ hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const Lau
 hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
                               hipStream_t st);
 hipError_t launch_reset_counter(int32_t* counter, hipStream_t st);
+hipError_t launch_rtab(float* rtab, int S, float image_size, hipStream_t st);
 hipError_t launch_fp32_batch(bool grid, const BatchArgs& b, int S, int workgroups, hipStream_t st);
 // kernel: p9idg_kernel; workgroups 0 = CUs x occupancy
 hipError_t launch_grid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st);
@@ -124,6 +126,9 @@ hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid
 hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t* flags,
                            float2* out, size_t n, hipStream_t st);
 hipError_t launch_zero_boxes(float2* base, const Box* boxes, int nboxes, int G, hipStream_t st);
+hipError_t launch_sum_layers(const float2* layers, int L, size_t GG, float2* out, hipStream_t st);
+hipError_t launch_sum_layer_boxes(const float2* layers, const Box* boxes, int nboxes, int G, Box u,
+                                  float2* out, hipStream_t st);
 hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st);
 
 }  // namespace p9idg
