@@ -268,6 +268,25 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "idg_mfma_bf16_large_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, direction):
+    """HBM bytes per launch of the large workload's kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith("hbm_bytes_per_launch:"):
+                    table = json.loads(line.split(":", 1)[1])
+                    v = table.get("%s/%s" % (kernel, direction))
+                    return None if v is None else int(v)
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def copy_probe(gib):
     """A plain copy kernel of known size, profiled beside the gridder to turn
     FETCH_SIZE / WRITE_SIZE counter readings into bytes."""
@@ -383,7 +402,7 @@ def main():
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
-        "traffic": None,
+        "traffic": profiled_traffic(args.kernel, dom_name),
     }
     cpu, err, full = None, None, None
     model_grid = plan.download_model_grid()
