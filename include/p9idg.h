@@ -135,6 +135,9 @@ int p9idg_partition(const p9idg_constants* c, const double* uvw, int32_t nr_base
 /* Replaces constructing GriddingConstants; owns device state.  IDisposable
  * pattern of Core/FFT.cs:49-76. */
 int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan);
+/* The plan's constants and device (no counterpart in the reference: the
+ * constants object is the caller's own there). */
+int p9idg_get_constants(const p9idg_plan* plan, p9idg_constants* c, int32_t* device);
 int p9idg_destroy(p9idg_plan* plan);
 int p9idg_set_kernel(p9idg_plan* plan, int32_t kernel);
 int p9idg_get_kernel(const p9idg_plan* plan, int32_t* kernel);

@@ -137,6 +137,50 @@ void orc_image_to_vis(const orc_constants* c, const orc_subgrid* meta, int64_t n
                       const float* img, const double* uvw, int32_t B, int32_t T,
                       const double* freq, int32_t C, double* vis_out);
 
+/* ---- serial major cycle (oracle_cycle.cpp) ---------------------------- */
+/* per-cycle figures of MajorCycle.cs:92-117 */
+typedef struct orc_cycle_stats {
+    float max_dirty;     /* Residuals.GetMax(dirtyImage) */
+    float max_gradient;  /* Residuals.GetMax(gradients) */
+    float lambda;        /* currentLambda */
+    float abs_max_diff;  /* deconvolver.GetAbsMaxDiff(xImage, gradients, lambdaTrue, alpha) */
+    double objective;    /* Residuals.CalcPenalty + ElasticNet.CalcPenalty */
+    int32_t iterations;  /* deconvolution iterations (0 when skipped) */
+    int32_t converged;
+} orc_cycle_stats;
+
+/* Common.cs:227-237, 206-218, 128-135, 48-126 */
+void orc_psf_cut(const float* psf, int32_t n, int32_t factor, float* out);
+float orc_psf_max_sidelobe(const float* psf, int32_t n, int32_t factor);
+double orc_psf_max_lipschitz(const float* psf, int32_t h, int32_t w);
+void orc_psf_amap(const float* psf, int32_t ph, int32_t pw, int32_t H, int32_t W, float* amap);
+/* Common.cs:146-160: complex (ypad + ph) x (xpad + pw) */
+void orc_psf_padded_correlation(const float* psf, int32_t ph, int32_t pw, int32_t ypad, int32_t xpad,
+                                double* out);
+/* PaddedConvolver.cs:44-66 */
+void orc_padded_convolve(const double* kernel, int32_t kh, int32_t kw, const float* image, int32_t H,
+                         int32_t W, float* out);
+/* Common.cs:182-194: 2ph x 2pw */
+void orc_psf_squared(const float* psf, int32_t ph, int32_t pw, float* out);
+/* Common.cs:244-260, 22-40 */
+float orc_residuals_get_max(const float* img, int64_t n);
+double orc_residuals_penalty(const float* img, int64_t n);
+double orc_elastic_penalty(const float* img, int64_t n, float lambda, float alpha);
+float orc_proximal(float x, float lipschitz, float lambda, float alpha);
+/* FastSerialCD.cs:132-135, 77-101 */
+float orc_cd_abs_max_diff(const float* amap, const float* x, const float* g, int32_t H, int32_t W,
+                          float lambda, float alpha);
+int orc_cd_deconvolve(const float* amap, const float* psf2, int32_t p2h, int32_t p2w, float* x, float* g,
+                      int32_t H, int32_t W, float lambda, float alpha, int32_t iterations, float epsilon,
+                      int32_t* iters_out);
+/* MajorCycle.cs:33-134 (FastSerialCD, no FITS output); stats[max_major] */
+int orc_reconstruct_serial_cd(const orc_constants* c, const orc_subgrid* meta, int64_t n,
+                              const double* vis, const uint8_t* flags, const double* uvw, int32_t B,
+                              int32_t T, const double* freq, int32_t C, int32_t psf_cut_factor,
+                              int32_t max_major, float lambda, float alpha, int32_t deconv_iterations,
+                              float deconv_epsilon, float* x_image, orc_cycle_stats* stats,
+                              int32_t* ncycles);
+
 #ifdef __cplusplus
 }
 #endif

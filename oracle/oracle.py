@@ -293,3 +293,152 @@ def image_to_vis(c, meta, img, uvw, freq):
     out = np.zeros((B, T, C), dtype=np.complex128)
     lib().orc_image_to_vis(_cc(c), _p(meta), ctypes.c_int64(len(meta)), _p(im), _p(uvw), B, T, _p(freq), C, _p(out))
     return out
+
+
+# ---- serial major cycle (oracle_cycle.cpp) ---------------------------------
+CYCLE_STATS_DTYPE = np.dtype(
+    [
+        ("max_dirty", "<f4"),
+        ("max_gradient", "<f4"),
+        ("lambda_", "<f4"),
+        ("abs_max_diff", "<f4"),
+        ("objective", "<f8"),
+        ("iterations", "<i4"),
+        ("converged", "<i4"),
+    ],
+    align=True,
+)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def psf_cut(psf, factor):
+    psf = _f32(psf)
+    n = psf.shape[0]
+    out = np.zeros((n // factor, n // factor), dtype=np.float32)
+    lib().orc_psf_cut(_p(psf), n, int(factor), _p(out))
+    return out
+
+
+def psf_max_sidelobe(psf, factor):
+    psf = _f32(psf)
+    f = lib().orc_psf_max_sidelobe
+    f.restype = ctypes.c_float
+    return f(_p(psf), psf.shape[0], int(factor))
+
+
+def psf_max_lipschitz(psf):
+    psf = _f32(psf)
+    f = lib().orc_psf_max_lipschitz
+    f.restype = ctypes.c_double
+    return f(_p(psf), psf.shape[0], psf.shape[1])
+
+
+def psf_amap(psf, H, W):
+    psf = _f32(psf)
+    out = np.zeros((H, W), dtype=np.float32)
+    lib().orc_psf_amap(_p(psf), psf.shape[0], psf.shape[1], int(H), int(W), _p(out))
+    return out
+
+
+def psf_padded_correlation(psf, ypad, xpad):
+    psf = _f32(psf)
+    ph, pw = psf.shape
+    out = np.zeros((ypad + ph, xpad + pw), dtype=np.complex128)
+    lib().orc_psf_padded_correlation(_p(psf), ph, pw, int(ypad), int(xpad), _p(out))
+    return out
+
+
+def padded_convolve(kernel, kh, kw, image):
+    image = _f32(image)
+    H, W = image.shape
+    k = np.ascontiguousarray(kernel, dtype=np.complex128)
+    out = np.zeros((H, W), dtype=np.float32)
+    lib().orc_padded_convolve(_p(k), int(kh), int(kw), _p(image), H, W, _p(out))
+    return out
+
+
+def psf_squared(psf):
+    psf = _f32(psf)
+    ph, pw = psf.shape
+    out = np.zeros((2 * ph, 2 * pw), dtype=np.float32)
+    lib().orc_psf_squared(_p(psf), ph, pw, _p(out))
+    return out
+
+
+def residuals_get_max(img):
+    img = _f32(img)
+    f = lib().orc_residuals_get_max
+    f.restype = ctypes.c_float
+    return f(_p(img), ctypes.c_int64(img.size))
+
+
+def residuals_penalty(img):
+    img = _f32(img)
+    f = lib().orc_residuals_penalty
+    f.restype = ctypes.c_double
+    return f(_p(img), ctypes.c_int64(img.size))
+
+
+def elastic_penalty(img, lam, alpha):
+    img = _f32(img)
+    f = lib().orc_elastic_penalty
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_float, ctypes.c_float]
+    return f(_p(img), img.size, lam, alpha)
+
+
+def proximal(x, lipschitz, lam, alpha):
+    f = lib().orc_proximal
+    f.restype = ctypes.c_float
+    f.argtypes = [ctypes.c_float] * 4
+    return f(x, lipschitz, lam, alpha)
+
+
+def cd_abs_max_diff(amap, x, g, lam, alpha):
+    amap, x, g = _f32(amap), _f32(x), _f32(g)
+    f = lib().orc_cd_abs_max_diff
+    f.restype = ctypes.c_float
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float]
+    return f(_p(amap), _p(x), _p(g), x.shape[0], x.shape[1], lam, alpha)
+
+
+def cd_deconvolve(amap, psf2, x, g, lam, alpha, iterations, epsilon):
+    """FastSerialCD.Deconvolve; returns (x, g, converged, iterations) with copies updated."""
+    amap, psf2 = _f32(amap), _f32(psf2)
+    x, g = np.array(x, dtype=np.float32, copy=True), np.array(g, dtype=np.float32, copy=True)
+    it = ctypes.c_int32(0)
+    f = lib().orc_cd_deconvolve
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_float,
+                  ctypes.c_void_p]
+    conv = f(_p(amap), _p(psf2), psf2.shape[0], psf2.shape[1], _p(x), _p(g), x.shape[0], x.shape[1], lam, alpha,
+             int(iterations), epsilon, ctypes.addressof(it))
+    return x, g, bool(conv), it.value
+
+
+def reconstruct_serial_cd(c, meta, vis, flags, uvw, freq, psf_cut_factor, max_major, lam, alpha,
+                          deconv_iterations, deconv_epsilon):
+    """MajorCycle.ReconstructSerialCD; returns (x_image, stats[ncycles])."""
+    B, T, C = _dims(uvw, freq)
+    G = c.grid_size
+    vis = np.ascontiguousarray(vis, dtype=np.complex128)
+    fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+    x = np.zeros((G, G), dtype=np.float32)
+    stats = np.zeros(max_major, dtype=CYCLE_STATS_DTYPE)
+    nc = ctypes.c_int32(0)
+    f = lib().orc_reconstruct_serial_cd
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                  ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    st = f(ctypes.addressof(c), _p(meta), len(meta), _p(vis), None if fl is None else _p(fl), _p(uvw), B, T,
+           _p(freq), C, int(psf_cut_factor), int(max_major), lam, alpha, int(deconv_iterations), deconv_epsilon,
+           _p(x), _p(stats), ctypes.addressof(nc))
+    if st:
+        raise OracleError(st)
+    return x, stats[: nc.value]

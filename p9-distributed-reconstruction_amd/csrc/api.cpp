@@ -685,6 +685,13 @@ int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
     return P9IDG_OK;
 }
 
+int p9idg_get_constants(const p9idg_plan* p, p9idg_constants* c, int32_t* device) {
+    if (int st = check_plan(p)) return st;
+    if (c) *c = p->c;
+    if (device) *device = p->device;
+    return P9IDG_OK;
+}
+
 int p9idg_destroy(p9idg_plan* p) {
     if (!p) return P9IDG_OK;
     DeviceGuard dg(p->device);

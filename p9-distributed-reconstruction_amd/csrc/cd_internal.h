@@ -1,0 +1,65 @@
+// Device side of the serial coordinate-descent deconvolver (FastSerialCD.cs)
+// and of the PSF correlation (PaddedConvolver.cs, Common.cs PSF helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace p9idg {
+namespace cd {
+
+constexpr int kTile = 32;  // edge of the argmax tiles (pixels)
+
+// A candidate update: |x - prox|, pixel index y * W + x, new and old value.
+struct Cand {
+    float diff;
+    int32_t idx;
+    float xnew;
+    float xold;
+};
+
+// Deconvolution state on the device, read by every kernel of an iteration.
+struct State {
+    int32_t done;       // no further updates (converged or iteration limit)
+    int32_t converged;  // the last selected |diff| was below epsilon
+    int32_t iters;      // updates made
+    int32_t updated;    // the last selection changed a pixel
+    Cand last;          // that change
+    float absmax;       // the last selected |diff|
+    int32_t pad[3];
+};
+
+// All tiles of the table for lambda / alpha.
+hipError_t launch_tiles_all(const float* x, const float* g, const float* amap, int H, int W, float lambda,
+                            float alpha, Cand* tiles, hipStream_t st);
+// Tiles that intersect the PSF^2 window of the last update.
+hipError_t launch_tiles_window(const float* x, const float* g, const float* amap, int H, int W, float lambda,
+                               float alpha, Cand* tiles, const State* state, int p2h, int p2w, hipStream_t st);
+// Best tile: converged / done bookkeeping and the pixel write (update = 0:
+// only record absmax).
+hipError_t launch_select(Cand* tiles, int ntiles, State* state, float* x, float epsilon, int max_iter,
+                         int update, hipStream_t st);
+// Gradient update of the last change inside the PSF^2 window.
+hipError_t launch_update_gradients(float* g, const float* psf2, int p2h, int p2w, int H, int W,
+                                   const State* state, hipStream_t st);
+
+// PaddedConvolver / CalcPaddedFourierCorrelation pieces (complex double
+// buffers of N0 x N1).
+hipError_t launch_place_correlation(const float* psf, int ph, int pw, int ypad, int xpad, int N0, int N1,
+                                    double2* buf, hipStream_t st);
+hipError_t launch_place_centered(const float* psf, int ph, int pw, int N0, int N1, double2* buf,
+                                 hipStream_t st);
+hipError_t launch_insert_image(const float* img, int H, int W, int yh, int xh, int N0, int N1, double2* buf,
+                               hipStream_t st);
+hipError_t launch_multiply(double2* buf, const double2* kernel, size_t n, hipStream_t st);
+// out[y][x] = (float)(Re buf[y + yh][x + xh] / norm)
+hipError_t launch_extract(const double2* buf, int N1, int yh, int xh, double norm, int H, int W, float* out,
+                          hipStream_t st);
+// Common.cs:48-126 per pixel from the prefix sums of psf^2
+hipError_t launch_amap(const double* scan, int ph, int pw, int H, int W, float* amap, hipStream_t st);
+// max(0, max_i v[i]) into *out (Residuals.GetMax)
+hipError_t launch_get_max(const float* v, size_t n, float* out, hipStream_t st);
+
+}  // namespace cd
+}  // namespace p9idg

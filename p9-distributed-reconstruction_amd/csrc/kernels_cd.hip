@@ -1,0 +1,321 @@
+// Serial coordinate descent (FastSerialCD.cs) and the PSF correlation pieces
+// (PaddedConvolver.cs, Common.cs:48-194) on the GPU.
+//
+// FastSerialCD picks, per iteration, the pixel whose elastic-net proximal
+// update changes it most (first row, then first column on ties), writes it
+// and adds psf2 * (old - new) to the gradients in the PSF^2 window.  Only the
+// window's pixels change, so the argmax is kept as a table of per-tile bests:
+// an iteration re-evaluates the tiles the window touches and reduces the
+// table.  Float expressions use one rounding per operation (no contraction),
+// as the reference's float code does (this file is built with
+// -ffp-contract=off).
+#include <climits>
+
+#include "cd_internal.h"
+
+namespace p9idg {
+namespace cd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSelectThreads = 1024;
+
+// Common.cs:27-29; Math.Max(v, 0f) gives +0 for -0
+__device__ __forceinline__ float prox(float x, float lipschitz, float lambda, float alpha) {
+    float num = __fsub_rn(x, __fmul_rn(lambda, alpha));
+    num = num > 0.f ? num : 0.f;
+    const float den = __fadd_rn(lipschitz, __fmul_rn(lambda, __fsub_rn(1.f, alpha)));
+    return __fdiv_rn(num, den);
+}
+
+// Larger |diff| wins; equal nonzero |diff|: the smaller pixel index (the
+// reference's row-then-column scan keeps the first maximum).
+__device__ __forceinline__ bool better(const Cand& a, const Cand& b) {
+    return a.diff > b.diff || (a.diff == b.diff && a.diff > 0.f && a.idx < b.idx);
+}
+
+__device__ __forceinline__ Cand shfl_xor(const Cand& c, int m) {
+    Cand o;
+    o.diff = __shfl_xor(c.diff, m);
+    o.idx = __shfl_xor(c.idx, m);
+    o.xnew = __shfl_xor(c.xnew, m);
+    o.xold = __shfl_xor(c.xold, m);
+    return o;
+}
+
+template <int NT>
+__device__ Cand block_best(Cand c, Cand* sh) {
+    for (int m = 32; m > 0; m >>= 1) {
+        const Cand o = shfl_xor(c, m);
+        if (better(o, c)) c = o;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; w++)
+            if (better(sh[w], c)) c = sh[w];
+        sh[0] = c;
+    }
+    __syncthreads();
+    return sh[0];
+}
+
+__device__ __forceinline__ Cand none() { return Cand{0.f, INT_MAX, 0.f, 0.f}; }
+
+// FastSerialCD.cs:150-162 for one pixel
+__device__ __forceinline__ Cand eval(const float* x, const float* g, const float* amap, int idx, float lambda,
+                                     float alpha) {
+    const float a = amap[idx], old = x[idx];
+    const float t = prox(__fadd_rn(__fmul_rn(old, a), g[idx]), a, lambda, alpha);
+    return Cand{fabsf(__fsub_rn(old, t)), idx, t, old};
+}
+
+__device__ void tile_best(const float* x, const float* g, const float* amap, int H, int W, int ty, int tx,
+                          float lambda, float alpha, Cand* tiles, int ntx) {
+    __shared__ Cand sh[kThreads / 64];
+    Cand best = none();
+    for (int p = threadIdx.x; p < kTile * kTile; p += kThreads) {
+        const int y = ty * kTile + p / kTile, xx = tx * kTile + p % kTile;
+        if (y < H && xx < W) {
+            const Cand c = eval(x, g, amap, y * W + xx, lambda, alpha);
+            if (better(c, best)) best = c;
+        }
+    }
+    best = block_best<kThreads>(best, sh);
+    if (threadIdx.x == 0) tiles[ty * ntx + tx] = best;
+}
+
+__global__ __launch_bounds__(kThreads) void tiles_all_kernel(const float* x, const float* g, const float* amap,
+                                                             int H, int W, float lambda, float alpha, Cand* tiles) {
+    tile_best(x, g, amap, H, W, blockIdx.y, blockIdx.x, lambda, alpha, tiles, gridDim.x);
+}
+
+__global__ __launch_bounds__(kThreads) void tiles_window_kernel(const float* x, const float* g, const float* amap,
+                                                                int H, int W, float lambda, float alpha, Cand* tiles,
+                                                                const State* st, int p2h, int p2w) {
+    if (!st->updated) return;
+    const int y0 = st->last.idx / W, x0 = st->last.idx % W;
+    const int r0 = max(y0 - p2h / 2, 0), r1 = min(y0 - p2h / 2 + p2h, H);
+    const int c0 = max(x0 - p2w / 2, 0), c1 = min(x0 - p2w / 2 + p2w, W);
+    const int ty = r0 / kTile + (int)blockIdx.y, tx = c0 / kTile + (int)blockIdx.x;
+    if (ty > (r1 - 1) / kTile || tx > (c1 - 1) / kTile) return;
+    tile_best(x, g, amap, H, W, ty, tx, lambda, alpha, tiles, (W + kTile - 1) / kTile);
+}
+
+// FastSerialCD.cs:84-97: converged test, pixel write and iteration count.
+__global__ __launch_bounds__(kSelectThreads) void select_kernel(const Cand* tiles, int ntiles, State* st, float* x,
+                                                                float epsilon, int max_iter, int update) {
+    __shared__ Cand sh[kSelectThreads / 64];
+    if (update && st->done) {
+        if (threadIdx.x == 0) st->updated = 0;
+        return;
+    }
+    Cand best = none();
+    for (int i = threadIdx.x; i < ntiles; i += kSelectThreads)
+        if (better(tiles[i], best)) best = tiles[i];
+    best = block_best<kSelectThreads>(best, sh);
+    if (threadIdx.x != 0) return;
+    st->absmax = best.diff;
+    if (!update) return;
+    if (best.diff < epsilon || best.idx == INT_MAX) {
+        st->converged = best.diff < epsilon;
+        st->done = 1;
+        st->updated = 0;
+        return;
+    }
+    x[best.idx] = best.xnew;
+    st->last = best;
+    st->updated = 1;
+    st->iters += 1;
+    if (st->iters >= max_iter) st->done = 1;
+}
+
+// FastSerialCD.cs:219-238
+__global__ __launch_bounds__(kThreads) void update_kernel(float* g, const float* psf2, int p2h, int p2w, int H, int W,
+                                                          const State* st) {
+    if (!st->updated) return;
+    const int y0 = st->last.idx / W, x0 = st->last.idx % W;
+    const float d = __fsub_rn(st->last.xold, st->last.xnew);
+    const int n = p2h * p2w;
+    for (int p = blockIdx.x * kThreads + threadIdx.x; p < n; p += gridDim.x * kThreads) {
+        const int i = y0 - p2h / 2 + p / p2w, j = x0 - p2w / 2 + p % p2w;
+        if (i >= 0 && i < H && j >= 0 && j < W) g[i * W + j] = __fadd_rn(g[i * W + j], __fmul_rn(psf2[p], d));
+    }
+}
+
+// Common.cs:146-160: flipped PSF at (y + ypad/2 + 1, x + xpad/2 + 1), then
+// FFT.Shift (quadrant swap, self-inverse for even N0 = N1)
+__global__ void place_correlation_kernel(const float* psf, int ph, int pw, int ypad, int xpad, int N0, int N1,
+                                         double2* buf) {
+    const size_t n = (size_t)N0 * N1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(i / N1), b = (int)(i % N1);
+        const int h = N0 / 2;
+        // quadrant swap of the first h x h block pairs (FFT.cs:248-264)
+        int p = a, q = b;
+        if (a < 2 * h && b < 2 * h) {
+            p = a < h ? a + h : a - h;
+            q = b < h ? b + h : b - h;
+        }
+        const int y = p - ypad / 2 - 1, x = q - xpad / 2 - 1;
+        double v = 0.0;
+        if (y >= 0 && y < ph && x >= 0 && x < pw) v = (double)psf[(ph - y - 1) * pw + (pw - x - 1)];
+        buf[i] = make_double2(v, 0.0);
+    }
+}
+
+// Common.cs:164-172 (SetPsfInWindow): psf at (i + ph/2, j + pw/2)
+__global__ void place_centered_kernel(const float* psf, int ph, int pw, int N0, int N1, double2* buf) {
+    const size_t n = (size_t)N0 * N1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / N1) - ph / 2, x = (int)(i % N1) - pw / 2;
+        const double v = (y >= 0 && y < ph && x >= 0 && x < pw) ? (double)psf[y * pw + x] : 0.0;
+        buf[i] = make_double2(v, 0.0);
+    }
+}
+
+// PaddedConvolver.cs:95-114
+__global__ void insert_image_kernel(const float* img, int H, int W, int yh, int xh, int N0, int N1, double2* buf) {
+    const size_t n = (size_t)N0 * N1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / N1) - yh, x = (int)(i % N1) - xh;
+        const double v = (y >= 0 && y < H && x >= 0 && x < W) ? (double)img[(size_t)y * W + x] : 0.0;
+        buf[i] = make_double2(v, 0.0);
+    }
+}
+
+__global__ void multiply_kernel(double2* buf, const double2* k, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double2 a = buf[i], b = k[i];
+        buf[i] = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+}
+
+// PaddedConvolver.cs:58-64
+__global__ void extract_kernel(const double2* buf, int N1, int yh, int xh, double norm, int H, int W, float* out) {
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i % W);
+        out[i] = (float)(buf[(size_t)(y + yh) * N1 + (x + xh)].x / norm);
+    }
+}
+
+// Common.cs:88-126 (QueryScan) on the prefix sums of psf^2
+__global__ void amap_kernel(const double* s, int ph, int pw, int H, int W, float* amap) {
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i % W);
+        const int y_over = max(0, y + (ph - ph / 2) - H), x_over = max(0, x + (pw - pw / 2) - W);
+        const int y_under = -(y - ph / 2), x_under = -(x - pw / 2);
+        const int yu = max(1, y_under) - 1, xu = max(1, x_under) - 1;
+        auto S = [&](int a, int b) { return s[a * pw + b]; };
+        double v;
+        if (y_over <= 0 && x_over <= 0 && y_under <= 0 && x_under <= 0) {
+            v = S(ph - 1, pw - 1);
+        } else if (y_under > 0 && x_under > 0) {
+            v = S(ph - 1, pw - 1) - S(ph - 1, xu) - S(yu, pw - 1) + S(yu, xu);
+        } else {
+            double corr = 0.0;
+            if (y_under > 0) corr += S(yu, pw - x_over - 1);
+            if (x_under > 0) corr += S(ph - y_over - 1, xu);
+            v = S(ph - 1 - y_over, pw - 1 - x_over) - corr;
+        }
+        amap[i] = (float)v;
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned* p) { *p = 0u; }
+
+// Residuals.GetMax (Common.cs:244-251): max(0, v) as unsigned bits
+__global__ void get_max_kernel(const float* v, size_t n, unsigned* out) {
+    float m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, v[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+unsigned blocks(size_t n) {
+    const size_t b = (n + kThreads - 1) / kThreads;
+    return (unsigned)(b > 4096 ? 4096 : (b == 0 ? 1 : b));
+}
+
+}  // namespace
+
+hipError_t launch_tiles_all(const float* x, const float* g, const float* amap, int H, int W, float lambda,
+                            float alpha, Cand* tiles, hipStream_t st) {
+    const dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
+    hipLaunchKernelGGL(tiles_all_kernel, grid, dim3(kThreads), 0, st, x, g, amap, H, W, lambda, alpha, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiles_window(const float* x, const float* g, const float* amap, int H, int W, float lambda,
+                               float alpha, Cand* tiles, const State* state, int p2h, int p2w, hipStream_t st) {
+    const dim3 grid((p2w + kTile - 1) / kTile + 1, (p2h + kTile - 1) / kTile + 1);
+    hipLaunchKernelGGL(tiles_window_kernel, grid, dim3(kThreads), 0, st, x, g, amap, H, W, lambda, alpha, tiles,
+                       state, p2h, p2w);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(Cand* tiles, int ntiles, State* state, float* x, float epsilon, int max_iter, int update,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(select_kernel, dim3(1), dim3(kSelectThreads), 0, st, tiles, ntiles, state, x, epsilon,
+                       max_iter, update);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_gradients(float* g, const float* psf2, int p2h, int p2w, int H, int W, const State* state,
+                                   hipStream_t st) {
+    hipLaunchKernelGGL(update_kernel, dim3(blocks((size_t)p2h * p2w)), dim3(kThreads), 0, st, g, psf2, p2h, p2w, H, W,
+                       state);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_correlation(const float* psf, int ph, int pw, int ypad, int xpad, int N0, int N1,
+                                    double2* buf, hipStream_t st) {
+    hipLaunchKernelGGL(place_correlation_kernel, dim3(blocks((size_t)N0 * N1)), dim3(kThreads), 0, st, psf, ph, pw,
+                       ypad, xpad, N0, N1, buf);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_centered(const float* psf, int ph, int pw, int N0, int N1, double2* buf, hipStream_t st) {
+    hipLaunchKernelGGL(place_centered_kernel, dim3(blocks((size_t)N0 * N1)), dim3(kThreads), 0, st, psf, ph, pw, N0,
+                       N1, buf);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_image(const float* img, int H, int W, int yh, int xh, int N0, int N1, double2* buf,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(insert_image_kernel, dim3(blocks((size_t)N0 * N1)), dim3(kThreads), 0, st, img, H, W, yh, xh,
+                       N0, N1, buf);
+    return hipGetLastError();
+}
+
+hipError_t launch_multiply(double2* buf, const double2* kernel, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(multiply_kernel, dim3(blocks(n)), dim3(kThreads), 0, st, buf, kernel, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract(const double2* buf, int N1, int yh, int xh, double norm, int H, int W, float* out,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(extract_kernel, dim3(blocks((size_t)H * W)), dim3(kThreads), 0, st, buf, N1, yh, xh, norm, H,
+                       W, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_amap(const double* scan, int ph, int pw, int H, int W, float* amap, hipStream_t st) {
+    hipLaunchKernelGGL(amap_kernel, dim3(blocks((size_t)H * W)), dim3(kThreads), 0, st, scan, ph, pw, H, W, amap);
+    return hipGetLastError();
+}
+
+hipError_t launch_get_max(const float* v, size_t n, float* out, hipStream_t st) {
+    unsigned* o = reinterpret_cast<unsigned*>(out);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, st, o);
+    hipLaunchKernelGGL(get_max_kernel, dim3(blocks(n)), dim3(kThreads), 0, st, v, n, o);
+    return hipGetLastError();
+}
+
+}  // namespace cd
+}  // namespace p9idg

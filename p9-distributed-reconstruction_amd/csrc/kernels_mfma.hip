@@ -686,12 +686,15 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
     }
 }
 
+// two waves per SIMD up to S = 48; the S = 64 gridder needs more registers
+template <int S>
+constexpr int kGridWaves = S <= 48 ? 2 : 1;
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void grid_mfma_kernel(KernelArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_kernel(KernelArgs a) {
     grid_mfma_body<S, P>(a, a.order[blockIdx.x], 0, -1);
 }
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void grid_mfma_queue_kernel(KernelArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_queue_kernel(KernelArgs a) {
     run_queue(a, [&](int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
 }
 
@@ -715,7 +718,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
 }
 
 template <int S, class P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void grid_mfma_batch_kernel(BatchArgs b) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_batch_kernel(BatchArgs b) {
     run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
 }
 template <int S, class P>

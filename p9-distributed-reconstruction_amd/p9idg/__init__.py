@@ -15,6 +15,7 @@ from .idg import (  # noqa: F401
     SUBGRID_DTYPE,
     Visibilities,
 )
+from .cycle import CYCLE_STATS_DTYPE, PSF, DeconvolutionResult, FastSerialCD, MajorCycle, PaddedConvolver  # noqa: F401
 from . import _lib  # noqa: F401
 from ._lib import (  # noqa: F401
     DEGRID_PLAIN,

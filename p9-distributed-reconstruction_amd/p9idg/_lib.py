@@ -92,6 +92,7 @@ IDG_SIGNATURES = {
     "p9idg_last_error": [],
     "p9idg_partition": [_vp, _vp, _i32, _i32, _vp, _i32, _vp, _i64, _pi64, _vp],
     "p9idg_create": [_vp, _i32, ctypes.POINTER(_vp)],
+    "p9idg_get_constants": [_vp, _vp, ctypes.POINTER(_i32)],
     "p9idg_destroy": [_vp],
     "p9idg_set_kernel": [_vp, _i32],
     "p9idg_get_kernel": [_vp, ctypes.POINTER(_i32)],
@@ -149,6 +150,29 @@ IDG_SIGNATURES = {
     "p9idg_batch_sync": [_vp],
     "p9idg_batch_timings": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
 }
+_f32 = ctypes.c_float
+_pf32 = ctypes.POINTER(ctypes.c_float)
+# p9cycle.h (same library)
+CYCLE_SIGNATURES = {
+    "p9psf_cut": [_vp, _i32, _i32, _vp],
+    "p9psf_max_sidelobe": [_vp, _i32, _i32, _pf32],
+    "p9psf_max_lipschitz": [_vp, _i32, _i32, ctypes.POINTER(_f64)],
+    "p9psf_squared": [_i32, _vp, _i32, _i32, _vp],
+    "p9psf_amap": [_i32, _vp, _i32, _i32, _i32, _i32, _vp],
+    "p9conv_create_correlation": [_i32, _vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)],
+    "p9conv_destroy": [_vp],
+    "p9conv_convolve": [_vp, _vp, _vp],
+    "p9conv_convolve_device": [_vp, _vp, _vp],
+    "p9cd_create": [_i32, _i32, _i32, _vp, _i32, _i32, ctypes.POINTER(_vp)],
+    "p9cd_destroy": [_vp],
+    "p9cd_max_lipschitz": [_vp, _pf32],
+    "p9cd_get_abs_max_diff": [_vp, _vp, _vp, _f32, _f32, _pf32],
+    "p9cd_get_abs_max_diff_device": [_vp, _vp, _vp, _f32, _f32, _pf32],
+    "p9cd_deconvolve": [_vp, _vp, _vp, _f32, _f32, _i32, _f32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    "p9cd_deconvolve_device": [_vp, _vp, _vp, _f32, _f32, _i32, _f32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    "p9idg_reconstruct_serial_cd": [_vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _i32, _f32, _vp, _vp,
+                                    ctypes.POINTER(_i32)],
+}
 _RESTYPES = {"p9idg_last_error": ctypes.c_char_p}
 
 OBS_SIGNATURES = {
@@ -174,10 +198,11 @@ def idg():
                 % (IDG_LIB_PATH, LIB_DIR)
             )
         lib = ctypes.CDLL(IDG_LIB_PATH)
-        for name, args in IDG_SIGNATURES.items():
-            fn = getattr(lib, name)
-            fn.argtypes = args
-            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        for table in (IDG_SIGNATURES, CYCLE_SIGNATURES):
+            for name, args in table.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPES.get(name, ctypes.c_int)
         _idg = lib
     return _idg
 

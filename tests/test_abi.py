@@ -30,6 +30,14 @@ def test_idg_library_exports_every_declared_function():
     assert set(names) == set(_lib.IDG_SIGNATURES)
 
 
+def test_cycle_functions_exported():
+    lib = _lib.idg()
+    names = declared("p9cycle.h", "p9(?:psf|conv|cd|idg)_")
+    assert len(names) == 17
+    assert [n for n in names if not hasattr(lib, n)] == []
+    assert set(names) == set(_lib.CYCLE_SIGNATURES)
+
+
 def test_obs_library_exports_every_declared_function():
     lib = _lib.obs()
     names = declared("p9obs.h", "p9obs_")

@@ -1,0 +1,108 @@
+"""GPU serial major cycle against the oracle (FastSerialCD.cs, PaddedConvolver.cs,
+Common.cs PSF helpers, MajorCycle.cs:33-134).
+
+The deconvolver is compared bit for bit: given the same PSF^2 and Lipschitz
+map (the GPU's own, handed to the oracle), the GPU's iterations must pick the
+same pixels in the same order and leave identical images and gradients.
+The FP64 FFT products (PSF^2, the padded correlation) agree with the oracle's
+FP64 FFTs to float rounding."""
+import numpy as np
+import pytest
+
+import p9idg
+from p9idg import observation as obs
+
+from conftest import oracle_constants
+
+pytestmark = pytest.mark.gpu
+
+
+def beam(n, width, rng):
+    """A PSF-like image: central peak plus weak sidelobes, peak at (n/2, n/2)."""
+    y, x = np.mgrid[0:n, 0:n] - n // 2
+    b = np.exp(-(x * x + y * y) / (2.0 * width * width)) + 0.05 * rng.standard_normal((n, n))
+    b[n // 2, n // 2] = 1.0
+    return b.astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [8, 16, 48])
+def test_psf_squared_and_amap(O, n):
+    rng = np.random.default_rng(n)
+    psf = beam(n, n / 6.0, rng)
+    p2 = p9idg.PSF.CalcPSFSquared(psf)
+    ref = O.psf_squared(psf)
+    assert np.abs(p2 - ref).max() <= 1e-6 * np.abs(ref).max()
+    for H, W in [(96, 96), (70, 83)]:
+        assert np.array_equal(p9idg.PSF.CalcAMap(psf, (H, W)), O.psf_amap(psf, H, W))
+
+
+@pytest.mark.parametrize("H,ph", [(64, 16), (96, 24), (256, 64)])
+def test_padded_convolver(O, H, ph):
+    rng = np.random.default_rng(H)
+    psf = beam(ph, ph / 6.0, rng)
+    img = rng.standard_normal((H, H)).astype(np.float32)
+    with p9idg.PaddedConvolver(psf, (H, H)) as conv:
+        out = conv.Convolve(img)
+    ref = O.padded_convolve(O.psf_padded_correlation(psf, H, H), ph, ph, img)
+    assert np.abs(out - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+CD_CASES = [
+    # image, psf, lambda, alpha, iterations, epsilon
+    (96, 16, 0.05, 0.01, 400, 1e-5),
+    (70, 12, 0.02, 0.2, 150, 1e-6),  # edge tiles; stops at the iteration limit
+    (128, 32, 0.5, 0.01, 2000, 1e-4),
+]
+
+
+@pytest.mark.parametrize("H,ph,lam,alpha,iters,eps", CD_CASES)
+def test_fast_serial_cd_matches_oracle_bit_for_bit(O, H, ph, lam, alpha, iters, eps):
+    rng = np.random.default_rng(H + ph)
+    psf = beam(ph, ph / 6.0, rng)
+    sky = np.zeros((H, H), np.float32)
+    for _ in range(12):
+        sky[rng.integers(4, H - 4), rng.integers(4, H - 4)] = rng.uniform(0.2, 1.0)
+    with p9idg.PaddedConvolver(psf, (H, H)) as conv:
+        dirty = conv.Convolve(sky) + 0.01 * rng.standard_normal((H, H)).astype(np.float32)
+        g0 = conv.Convolve(dirty)
+    psf2 = p9idg.PSF.CalcPSFSquared(psf)
+    amap = p9idg.PSF.CalcAMap(psf, (H, H))
+    with p9idg.FastSerialCD((H, H), psf) as cd:
+        assert cd.MaxLipschitz == psf2.max()
+        x0 = np.zeros((H, H), np.float32)
+        assert cd.GetAbsMaxDiff(x0, g0, lam, alpha) == O.cd_abs_max_diff(amap, x0, g0, lam, alpha)
+        x = x0.copy()
+        g = g0.copy()
+        res = cd.Deconvolve(x, g, lam, alpha, iters, eps)
+    xr, gr, conv_r, it_r = O.cd_deconvolve(amap, psf2, x0, g0, lam, alpha, iters, eps)
+    print("H %d psf %d: %d iterations, converged %s" % (H, ph, res.IterationCount, res.Converged))
+    assert res.IterationCount == it_r and res.Converged == conv_r
+    assert res.IterationCount > 0
+    assert np.array_equal(x, xr)
+    assert np.array_equal(g, gr)
+
+
+@pytest.mark.parametrize("kernel", ["fp32", "mfma_bf16"])
+def test_reconstruct_serial_cd_against_oracle(O, kernel):
+    cell = obs.ARCSEC * 20.0
+    ob = obs.custom(G=128, S=16, K=4, L=4, n_ant=10, T=40, C=4, seed=95, core_diam=800.0, cell_arcsec=20.0,
+                    sources=[(1.0, 9 * cell, -5 * cell, 0.0), (0.6, -20 * cell, 14 * cell, 0.0),
+                             (0.3, 3 * cell, 25 * cell, 0.0)], noise=0.01)
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    args = (4, 3, 1.0, 0.01, 3000, 1e-5)
+    xr, sr = O.reconstruct_serial_cd(oc, part.subgrids, ob.vis, None, ob.uvw, ob.freq, *args)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        x, s = p9idg.MajorCycle.ReconstructSerialCD(plan, ob.vis, None, *args)
+    print(kernel, "cycles", len(s), len(sr))
+    print("gpu", s[["max_dirty", "lambda_", "abs_max_diff", "iterations"]])
+    print("ref", sr[["max_dirty", "lambda_", "abs_max_diff", "iterations"]])
+    assert len(s) == len(sr)
+    assert np.array_equal(s["iterations"], sr["iterations"])
+    assert np.allclose(s["max_dirty"], sr["max_dirty"], rtol=1e-5)
+    assert np.allclose(s["lambda_"], sr["lambda_"], rtol=1e-5)
+    err = np.abs(x - xr).max() / np.abs(xr).max()
+    print("model image max rel err %.2e" % err)
+    assert err <= 1e-5
+    assert np.unravel_index(np.argmax(x), x.shape) == np.unravel_index(np.argmax(xr), xr.shape)
