@@ -189,6 +189,115 @@ def full_accuracy(ob, part, model_grid_c64, kernel):
     return out
 
 
+def run_cycle(args):
+    """The large workload plus 10 major cycles of the serial coordinate-descent
+    deconvolver (MajorCycle.ReconstructSerialCD with the reference's LMC
+    settings: psf cut 16, lambda 1.0, alpha 0.01, 30000 iterations, 1e-5;
+    RunningMethod.cs:41-64), everything resident in HBM."""
+    import p9idg
+    from p9idg import observation
+
+    cut, cycles, lam, alpha, iters, eps = 16, 10, 1.0, 0.01, 30000, 1e-5
+    ob = observation.large(threads=host_threads())
+    part = ob.partition()
+    c = ob.c
+    G = c.GridSize
+    plan = p9idg.Plan(c, device=0, kernel=args.kernel)
+    plan.set_observation(ob.uvw, ob.freq, part)
+    plan.upload_vis(ob.vis)
+    if not args.no_autotune:
+        log("autotuned launch shape: %s" % plan.autotune(reps=3))
+    t = time.perf_counter()
+    x, stats = p9idg.MajorCycle.ReconstructSerialCD(plan, ob.vis, None, cut, cycles, lam, alpha, iters, eps)
+    total = time.perf_counter() - t
+    n_vis = plan.gridded_visibilities
+    for k, s in enumerate(stats):
+        log("cycle %d: lambda %.4g, |dx|max %.3g, %d iterations (converged %d), gradients %.1f ms, "
+            "deconvolution %.1f ms, imaging %.1f ms" % (k, s["lambda_"], s["abs_max_diff"], s["iterations"],
+                                                       s["converged"], s["gradient_ms"], s["deconv_ms"],
+                                                       s["imaging_ms"]))
+    total_iters = int(stats["iterations"].sum())
+    cd_ms = float(stats["deconv_ms"].sum())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # oracle costs on bounded samples of the same work: FastSerialCD
+        # iterations on the full image (same PSF cut) and the FP64 grid +
+        # degrid rate on a stride sample of subgrids
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        threads = host_threads()
+        O.set_threads(threads)
+        plan.set_psf_vis(None)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        psf = plan.download_image()
+        psf_cut = O.psf_cut(psf, cut)
+        pc = psf_cut.shape[0]
+        psf2 = O.psf_squared(psf_cut)
+        amap = O.psf_amap(psf_cut, G, G)
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        dirty = plan.download_image()
+        with p9idg.PaddedConvolver(psf_cut, (G, G)) as conv:
+            g0 = conv.Convolve(dirty)
+        k = 40
+        t0 = time.perf_counter()
+        O.cd_deconvolve(amap, psf2, np.zeros((G, G), np.float32), g0, float(stats[0]["lambda_"]), alpha, k, eps)
+        t_iter = (time.perf_counter() - t0) / k
+        oc = O.make_constants(c.VisibilitiesCount, G, c.SubgridSize, c.KernelSize, c.MaxTimestepsPerSubgrid,
+                              float(c.ScaleArcSec), c.WLayerCount, c.WStep)
+        sample = sample_subgrids(part.subgrids, len(ob.freq), 1_000_000)
+        nv = int(sample["timeSampleCount"].sum()) * len(ob.freq)
+        vis128 = ob.vis.astype(np.complex128)
+        t0 = time.perf_counter()
+        layers = O.gridw(oc, sample, vis128, ob.uvw, ob.freq)
+        O.degridw(oc, sample, layers.sum(axis=0), ob.uvw, ob.freq)
+        t_vis = (time.perf_counter() - t0) / nv
+        del layers
+        est = total_iters * t_iter + len(stats) * n_vis * t_vis
+        cpu = {
+            "value": est,
+            "unit": "s",
+            "cores": threads,
+            "kind": "port",
+            "sample": "FastSerialCD: %d iterations on the 3072^2 image (%.1f ms each, 1 thread); grid + degrid: %d "
+                      "visibilities (%.2f us each, %d threads); scaled to %d iterations and %d x %d visibilities"
+                      % (k, t_iter * 1e3, nv, t_vis * 1e6, threads, total_iters, len(stats), n_vis),
+        }
+    plan.close()
+    out = {
+        "metric": "seconds for 10 serial-CD major cycles, 3072^2 32 w-layers (MajorCycle.ReconstructSerialCD)",
+        "value": total,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": len(stats),
+        "warmup": 0,
+        "ms_per_step": total * 1e3 / max(len(stats), 1),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.kernel] + "; deconvolution f32, PSF correlation f64",
+        "data": "synthetic (seeded; see p9idg/observation.py 'large')",
+        "config": {
+            "workload": "large + major cycles: psf cut 16, lambda 1.0, alpha 0.01, 30000 iterations, epsilon 1e-5",
+            "kernel": args.kernel,
+            "cycles_run": len(stats),
+            "deconvolution_iterations": total_iters,
+            "deconvolution_ms": cd_ms,
+            "us_per_iteration": cd_ms * 1e3 / max(total_iters, 1),
+            "gridded_visibilities": int(n_vis),
+        },
+        "cycles": [{k: (float(s[k]) if s.dtype[k].kind == "f" else int(s[k])) for k in
+                    ("lambda_", "abs_max_diff", "iterations", "converged", "gradient_ms", "deconv_ms", "imaging_ms")}
+                   for s in stats],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args):
     """The 8 uneven seeded observations (p9idg/observation.py batch_instance)
     gridded and degridded as one persistent launch per subgrid-size class,
@@ -317,7 +426,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--workload", default="large", choices=["large", "batch"])
+    ap.add_argument("--workload", default="large", choices=["large", "batch", "cycle"])
     ap.add_argument("--full-accuracy", action="store_true",
                     help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
@@ -332,6 +441,8 @@ def main():
         copy_probe(args.copy_probe)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "cycle":
+        return run_cycle(args)
 
     import p9idg
     from p9idg import observation

@@ -33,16 +33,18 @@ struct State {
 // All tiles of the table for lambda / alpha.
 hipError_t launch_tiles_all(const float* x, const float* g, const float* amap, int H, int W, float lambda,
                             float alpha, Cand* tiles, hipStream_t st);
-// Tiles that intersect the PSF^2 window of the last update.
-hipError_t launch_tiles_window(const float* x, const float* g, const float* amap, int H, int W, float lambda,
-                               float alpha, Cand* tiles, const State* state, int p2h, int p2w, hipStream_t st);
-// Best tile: converged / done bookkeeping and the pixel write (update = 0:
-// only record absmax).
-hipError_t launch_select(Cand* tiles, int ntiles, State* state, float* x, float epsilon, int max_iter,
-                         int update, hipStream_t st);
-// Gradient update of the last change inside the PSF^2 window.
-hipError_t launch_update_gradients(float* g, const float* psf2, int p2h, int p2w, int H, int W,
-                                   const State* state, hipStream_t st);
+// GetAbsMaxDiff: the best |diff| of the tile table into state->absmax.
+hipError_t launch_absmax(const Cand* tiles, int ntiles, State* state, hipStream_t st);
+
+// The row table (best tile per tile row) and the first selection after
+// launch_tiles_all; then launch_iteration repeatedly: the window tiles take
+// the gradient change and are re-evaluated, and one workgroup refreshes the
+// rows the window spans and selects the next update.
+hipError_t launch_first_select(const Cand* tiles, Cand* rows, int H, int W, State* state, float* x, float epsilon,
+                               int max_iter, hipStream_t st);
+hipError_t launch_iteration(float* x, float* g, const float* amap, const float* psf2, int p2h, int p2w, int H, int W,
+                            float lambda, float alpha, Cand* tiles, Cand* rows, State* state, float epsilon,
+                            int max_iter, hipStream_t st);
 
 // PaddedConvolver / CalcPaddedFourierCorrelation pieces (complex double
 // buffers of N0 x N1).

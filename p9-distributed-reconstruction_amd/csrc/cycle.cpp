@@ -325,7 +325,9 @@ struct p9cd {
     float* d_amap = nullptr;
     float* d_psf2 = nullptr;
     cd::Cand* d_tiles = nullptr;
+    cd::Cand* d_rows = nullptr;
     cd::State* d_state = nullptr;
+
     float* d_io = nullptr;  // [2][H][W] staging for host-array calls
 };
 
@@ -335,7 +337,7 @@ constexpr int kBatch = 128;  // iterations enqueued between host checks of the s
 
 int cd_abs_max(p9cd* c, const float* d_x, const float* d_g, float lambda, float alpha, float* out) {
     P9_HIP(cd::launch_tiles_all(d_x, d_g, c->d_amap, c->H, c->W, lambda, alpha, c->d_tiles, c->stream));
-    P9_HIP(cd::launch_select(c->d_tiles, c->ntiles, c->d_state, nullptr, 0.f, 0, 0, c->stream));
+    P9_HIP(cd::launch_absmax(c->d_tiles, c->ntiles, c->d_state, c->stream));
     cd::State s;
     P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
     P9_HIP(hipStreamSynchronize(c->stream));
@@ -343,22 +345,40 @@ int cd_abs_max(p9cd* c, const float* d_x, const float* d_g, float lambda, float 
     return P9IDG_OK;
 }
 
+// Iterations are recorded once per call as a graph of kBatch iterations and
+// replayed until the state says done (launches after that return at once).
 int cd_run(p9cd* c, float* d_x, float* d_g, float lambda, float alpha, int32_t max_iter, float eps, int32_t* conv,
            int32_t* iters) {
     cd::State s{};
     s.done = max_iter <= 0;
     P9_HIP(hipMemcpyAsync(c->d_state, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
     P9_HIP(cd::launch_tiles_all(d_x, d_g, c->d_amap, c->H, c->W, lambda, alpha, c->d_tiles, c->stream));
-    while (!s.done) {
-        for (int k = 0; k < kBatch; k++) {
-            P9_HIP(cd::launch_select(c->d_tiles, c->ntiles, c->d_state, d_x, eps, max_iter, 1, c->stream));
-            P9_HIP(cd::launch_update_gradients(d_g, c->d_psf2, c->p2h, c->p2w, c->H, c->W, c->d_state, c->stream));
-            P9_HIP(cd::launch_tiles_window(d_x, d_g, c->d_amap, c->H, c->W, lambda, alpha, c->d_tiles, c->d_state,
-                                           c->p2h, c->p2w, c->stream));
-        }
-        P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
-        P9_HIP(hipStreamSynchronize(c->stream));
+    P9_HIP(cd::launch_first_select(c->d_tiles, c->d_rows, c->H, c->W, c->d_state, d_x, eps, max_iter, c->stream));
+    P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    P9_HIP(hipStreamSynchronize(c->stream));
+    if (s.done && !s.updated) {
+        if (conv) *conv = s.converged;
+        if (iters) *iters = s.iters;
+        return P9IDG_OK;
     }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    P9_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < kBatch && e == hipSuccess; k++)
+        e = cd::launch_iteration(d_x, d_g, c->d_amap, c->d_psf2, c->p2h, c->p2w, c->H, c->W, lambda, alpha, c->d_tiles,
+                                 c->d_rows, c->d_state, eps, max_iter, c->stream);
+    const hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
+    if (e == hipSuccess) e = e_end;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    while (e == hipSuccess && (!s.done || s.updated)) {
+        e = hipGraphLaunch(exec, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "deconvolution iterations");
     if (conv) *conv = s.converged;
     if (iters) *iters = s.iters;
     return P9IDG_OK;
@@ -390,6 +410,7 @@ int p9cd_create(int32_t device, int32_t height, int32_t width, const float* psf,
     int st;
     if ((st = alloc(&c->d_amap, (size_t)height * width)) || (st = alloc(&c->d_psf2, (size_t)c->p2h * c->p2w)) ||
         (st = alloc(&c->d_tiles, (size_t)c->ntiles)) || (st = alloc(&c->d_state, 1)) ||
+        (st = alloc(&c->d_rows, (size_t)(height + cd::kTile - 1) / cd::kTile)) ||
         (st = alloc(&c->d_io, 2 * (size_t)height * width)))
         return bail(st);
     // psf2 = PSF.CalcPSFSquared(psf), MaxLipschitz = Residuals.GetMax(psf2)
@@ -412,6 +433,7 @@ int p9cd_destroy(p9cd* c) {
     release(c->d_amap);
     release(c->d_psf2);
     release(c->d_tiles);
+    release(c->d_rows);
     release(c->d_state);
     release(c->d_io);
     if (c->stream) (void)hipStreamDestroy(c->stream);

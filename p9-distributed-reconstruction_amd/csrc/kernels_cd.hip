@@ -92,57 +92,14 @@ __global__ __launch_bounds__(kThreads) void tiles_all_kernel(const float* x, con
     tile_best(x, g, amap, H, W, blockIdx.y, blockIdx.x, lambda, alpha, tiles, gridDim.x);
 }
 
-__global__ __launch_bounds__(kThreads) void tiles_window_kernel(const float* x, const float* g, const float* amap,
-                                                                int H, int W, float lambda, float alpha, Cand* tiles,
-                                                                const State* st, int p2h, int p2w) {
-    if (!st->updated) return;
-    const int y0 = st->last.idx / W, x0 = st->last.idx % W;
-    const int r0 = max(y0 - p2h / 2, 0), r1 = min(y0 - p2h / 2 + p2h, H);
-    const int c0 = max(x0 - p2w / 2, 0), c1 = min(x0 - p2w / 2 + p2w, W);
-    const int ty = r0 / kTile + (int)blockIdx.y, tx = c0 / kTile + (int)blockIdx.x;
-    if (ty > (r1 - 1) / kTile || tx > (c1 - 1) / kTile) return;
-    tile_best(x, g, amap, H, W, ty, tx, lambda, alpha, tiles, (W + kTile - 1) / kTile);
-}
-
-// FastSerialCD.cs:84-97: converged test, pixel write and iteration count.
-__global__ __launch_bounds__(kSelectThreads) void select_kernel(const Cand* tiles, int ntiles, State* st, float* x,
-                                                                float epsilon, int max_iter, int update) {
+// FastSerialCD.cs:132-135 (GetAbsMaxDiff): the best |diff| of the tile table.
+__global__ __launch_bounds__(kSelectThreads) void absmax_kernel(const Cand* tiles, int ntiles, State* st) {
     __shared__ Cand sh[kSelectThreads / 64];
-    if (update && st->done) {
-        if (threadIdx.x == 0) st->updated = 0;
-        return;
-    }
     Cand best = none();
     for (int i = threadIdx.x; i < ntiles; i += kSelectThreads)
         if (better(tiles[i], best)) best = tiles[i];
     best = block_best<kSelectThreads>(best, sh);
-    if (threadIdx.x != 0) return;
-    st->absmax = best.diff;
-    if (!update) return;
-    if (best.diff < epsilon || best.idx == INT_MAX) {
-        st->converged = best.diff < epsilon;
-        st->done = 1;
-        st->updated = 0;
-        return;
-    }
-    x[best.idx] = best.xnew;
-    st->last = best;
-    st->updated = 1;
-    st->iters += 1;
-    if (st->iters >= max_iter) st->done = 1;
-}
-
-// FastSerialCD.cs:219-238
-__global__ __launch_bounds__(kThreads) void update_kernel(float* g, const float* psf2, int p2h, int p2w, int H, int W,
-                                                          const State* st) {
-    if (!st->updated) return;
-    const int y0 = st->last.idx / W, x0 = st->last.idx % W;
-    const float d = __fsub_rn(st->last.xold, st->last.xnew);
-    const int n = p2h * p2w;
-    for (int p = blockIdx.x * kThreads + threadIdx.x; p < n; p += gridDim.x * kThreads) {
-        const int i = y0 - p2h / 2 + p / p2w, j = x0 - p2w / 2 + p % p2w;
-        if (i >= 0 && i < H && j >= 0 && j < W) g[i * W + j] = __fadd_rn(g[i * W + j], __fmul_rn(psf2[p], d));
-    }
+    if (threadIdx.x == 0) st->absmax = best.diff;
 }
 
 // Common.cs:146-160: flipped PSF at (y + ypad/2 + 1, x + xpad/2 + 1), then
@@ -226,6 +183,104 @@ __global__ void amap_kernel(const double* s, int ph, int pw, int H, int W, float
     }
 }
 
+// --- one launch per iteration ------------------------------------------------
+// Best of tiles[first .. first + n) by one wave (64 lanes).
+__device__ Cand wave_best(const Cand* v, int n, int lane) {
+    Cand b = none();
+    for (int i = lane; i < n; i += 64)
+        if (better(v[i], b)) b = v[i];
+    for (int m = 32; m > 0; m >>= 1) {
+        const Cand o = shfl_xor(b, m);
+        if (better(o, b)) b = o;
+    }
+    return b;
+}
+
+// Rows [ty0, ty1] of the row table from the tile table, then the overall best
+// (4 waves), and the selection of FastSerialCD.cs:84-97 by thread 0.
+__device__ void rows_and_select(const Cand* tiles, Cand* rows, int ntx, int nty, int ty0, int ty1, State* st,
+                                float* x, float epsilon, int max_iter) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int ty = ty0 + wave; ty <= ty1; ty += kThreads / 64) {
+        const Cand b = wave_best(tiles + (size_t)ty * ntx, ntx, lane);
+        if (lane == 0) rows[ty] = b;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (wave != 0) return;
+    const Cand best = wave_best(rows, nty, lane);
+    if (lane != 0) return;
+    st->absmax = best.diff;
+    if (best.diff < epsilon || best.idx == INT_MAX) {
+        st->converged = best.diff < epsilon;
+        st->done = 1;
+        st->updated = 0;
+        return;
+    }
+    x[best.idx] = best.xnew;
+    st->last = best;
+    st->updated = 1;
+    st->iters += 1;
+    if (st->iters >= max_iter) st->done = 1;
+}
+
+// All rows and the first selection (after tiles_all_kernel).
+__global__ __launch_bounds__(kThreads) void first_select_kernel(const Cand* tiles, Cand* rows, int ntx, int nty,
+                                                                State* st, float* x, float epsilon, int max_iter) {
+    if (st->done) return;
+    rows_and_select(tiles, rows, ntx, nty, 0, nty - 1, st, x, epsilon, max_iter);
+}
+
+// Window half of an iteration: each block owns one tile of the PSF^2 window
+// of the last update, adds psf2 * (old - new) to its gradients
+// (FastSerialCD.cs:219-238) and re-evaluates the tile.  The change of the
+// last selection is applied even when that selection reached the iteration
+// limit (the reference updates before it stops).
+__global__ __launch_bounds__(kThreads) void window_kernel(float* x, float* g, const float* amap, const float* psf2,
+                                                          int p2h, int p2w, int H, int W, float lambda, float alpha,
+                                                          Cand* tiles, const State* st) {
+    __shared__ Cand sh[kThreads / 64];
+    if (!st->updated) return;  // uniform
+    const int ntx = (W + kTile - 1) / kTile;
+    const int y0 = st->last.idx / W, x0 = st->last.idx % W;
+    const float d = __fsub_rn(st->last.xold, st->last.xnew);
+    const int yh = p2h / 2, xh = p2w / 2;
+    const int r0 = max(y0 - yh, 0), r1 = min(y0 - yh + p2h, H);
+    const int c0 = max(x0 - xh, 0), c1 = min(x0 - xh + p2w, W);
+    const int ty = r0 / kTile + (int)blockIdx.y, tx = c0 / kTile + (int)blockIdx.x;
+    if (ty > (r1 - 1) / kTile || tx > (c1 - 1) / kTile) return;
+    Cand best = none();
+    for (int p = threadIdx.x; p < kTile * kTile; p += kThreads) {
+        const int i = ty * kTile + p / kTile, j = tx * kTile + p % kTile;
+        if (i < H && j < W) {
+            const int idx = i * W + j;
+            if (i >= r0 && i < r1 && j >= c0 && j < c1)
+                g[idx] = __fadd_rn(g[idx], __fmul_rn(psf2[(i - y0 + yh) * p2w + (j - x0 + xh)], d));
+            const Cand c = eval(x, g, amap, idx, lambda, alpha);
+            if (better(c, best)) best = c;
+        }
+    }
+    best = block_best<kThreads>(best, sh);
+    if (threadIdx.x == 0) tiles[ty * ntx + tx] = best;
+}
+
+// Selection half: refresh the rows the window spans and select the next
+// update (one workgroup).
+__global__ __launch_bounds__(kThreads) void row_select_kernel(const Cand* tiles, Cand* rows, int p2h, int H, int W,
+                                                              State* st, float* x, float epsilon, int max_iter) {
+    if (!st->updated) return;  // uniform
+    if (st->done) {
+        __syncthreads();
+        if (threadIdx.x == 0) st->updated = 0;
+        return;
+    }
+    const int y0 = st->last.idx / W;
+    const int r0 = max(y0 - p2h / 2, 0), r1 = min(y0 - p2h / 2 + p2h, H);
+    __syncthreads();  // every thread has read the state before thread 0 rewrites it
+    rows_and_select(tiles, rows, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile, r0 / kTile, (r1 - 1) / kTile, st, x,
+                    epsilon, max_iter);
+}
+
 __global__ void zero_u32_kernel(unsigned* p) { *p = 0u; }
 
 // Residuals.GetMax (Common.cs:244-251): max(0, v) as unsigned bits
@@ -251,25 +306,8 @@ hipError_t launch_tiles_all(const float* x, const float* g, const float* amap, i
     return hipGetLastError();
 }
 
-hipError_t launch_tiles_window(const float* x, const float* g, const float* amap, int H, int W, float lambda,
-                               float alpha, Cand* tiles, const State* state, int p2h, int p2w, hipStream_t st) {
-    const dim3 grid((p2w + kTile - 1) / kTile + 1, (p2h + kTile - 1) / kTile + 1);
-    hipLaunchKernelGGL(tiles_window_kernel, grid, dim3(kThreads), 0, st, x, g, amap, H, W, lambda, alpha, tiles,
-                       state, p2h, p2w);
-    return hipGetLastError();
-}
-
-hipError_t launch_select(Cand* tiles, int ntiles, State* state, float* x, float epsilon, int max_iter, int update,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(select_kernel, dim3(1), dim3(kSelectThreads), 0, st, tiles, ntiles, state, x, epsilon,
-                       max_iter, update);
-    return hipGetLastError();
-}
-
-hipError_t launch_update_gradients(float* g, const float* psf2, int p2h, int p2w, int H, int W, const State* state,
-                                   hipStream_t st) {
-    hipLaunchKernelGGL(update_kernel, dim3(blocks((size_t)p2h * p2w)), dim3(kThreads), 0, st, g, psf2, p2h, p2w, H, W,
-                       state);
+hipError_t launch_absmax(const Cand* tiles, int ntiles, State* state, hipStream_t st) {
+    hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(kSelectThreads), 0, st, tiles, ntiles, state);
     return hipGetLastError();
 }
 
@@ -302,6 +340,24 @@ hipError_t launch_extract(const double2* buf, int N1, int yh, int xh, double nor
                           hipStream_t st) {
     hipLaunchKernelGGL(extract_kernel, dim3(blocks((size_t)H * W)), dim3(kThreads), 0, st, buf, N1, yh, xh, norm, H,
                        W, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_first_select(const Cand* tiles, Cand* rows, int H, int W, State* state, float* x, float epsilon,
+                               int max_iter, hipStream_t st) {
+    hipLaunchKernelGGL(first_select_kernel, dim3(1), dim3(kThreads), 0, st, tiles, rows, (W + kTile - 1) / kTile,
+                       (H + kTile - 1) / kTile, state, x, epsilon, max_iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_iteration(float* x, float* g, const float* amap, const float* psf2, int p2h, int p2w, int H, int W,
+                            float lambda, float alpha, Cand* tiles, Cand* rows, State* state, float epsilon,
+                            int max_iter, hipStream_t st) {
+    const dim3 grid((p2w + kTile - 1) / kTile + 1, (p2h + kTile - 1) / kTile + 1);
+    hipLaunchKernelGGL(window_kernel, grid, dim3(kThreads), 0, st, x, g, amap, psf2, p2h, p2w, H, W, lambda, alpha,
+                       tiles, state);
+    hipLaunchKernelGGL(row_select_kernel, dim3(1), dim3(kThreads), 0, st, tiles, rows, p2h, H, W, state, x, epsilon,
+                       max_iter);
     return hipGetLastError();
 }
 
