@@ -197,7 +197,7 @@ def run_cycle(args):
     import p9idg
     from p9idg import observation
 
-    cut, cycles, lam, alpha, iters, eps = 16, 10, 1.0, 0.01, 30000, 1e-5
+    cut, cycles, lam, alpha, iters, eps = 16, args.cycles, 1.0, 0.01, 30000, 1e-5
     ob = observation.large(threads=host_threads())
     part = ob.partition()
     c = ob.c
@@ -269,7 +269,7 @@ def run_cycle(args):
         }
     plan.close()
     out = {
-        "metric": "seconds for 10 serial-CD major cycles, 3072^2 32 w-layers (MajorCycle.ReconstructSerialCD)",
+        "metric": "seconds for %d serial-CD major cycles, 3072^2 32 w-layers (MajorCycle.ReconstructSerialCD)" % cycles,
         "value": total,
         "unit": "s",
         "n_gpus": 1,
@@ -296,6 +296,35 @@ def run_cycle(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_cd(args):
+    """FastSerialCD alone on a 3072^2 image with a 192^2 PSF-like kernel (a
+    Gaussian main lobe with 2% noise sidelobes) and 400 point sources:
+    microseconds per deconvolution iteration."""
+    import p9idg
+
+    G, pc = 3072, 192
+    rng = np.random.default_rng(0)
+    y, x = np.mgrid[0:pc, 0:pc] - pc // 2
+    psf = (np.exp(-(x * x + y * y) / (2.0 * 12.0 ** 2)) + 0.02 * rng.standard_normal((pc, pc))).astype(np.float32)
+    psf[pc // 2, pc // 2] = 1.0
+    g = np.zeros((G, G), np.float32)
+    for _ in range(400):
+        g[rng.integers(100, G - 100), rng.integers(100, G - 100)] = rng.uniform(1, 50)
+    g += rng.standard_normal((G, G)).astype(np.float32) * 0.1
+    res = {}
+    with p9idg.FastSerialCD((G, G), psf) as cd:
+        for it in (args.warmup * 100, args.steps * 300):
+            xx = np.zeros((G, G), np.float32)
+            gg = g.copy()
+            t = time.perf_counter()
+            r = cd.Deconvolve(xx, gg, 20.0, 0.01, it, 1e-6)
+            res = {"iterations": r.IterationCount, "seconds": time.perf_counter() - t}
+    us = res["seconds"] / max(res["iterations"], 1) * 1e6
+    print(json.dumps({"metric": "us per FastSerialCD iteration, 3072^2 image, 192^2 PSF", "value": us,
+                      "unit": "us", "n_gpus": 1, "steps": res["iterations"], "higher_is_better": False,
+                      "data": "synthetic", "dtype": "f32"}), flush=True)
 
 
 def run_batch(args):
@@ -426,7 +455,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--workload", default="large", choices=["large", "batch", "cycle"])
+    ap.add_argument("--workload", default="large", choices=["large", "batch", "cycle", "cd"])
+    ap.add_argument("--cycles", type=int, default=10, help="major cycles of --workload cycle")
     ap.add_argument("--full-accuracy", action="store_true",
                     help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
@@ -443,6 +473,8 @@ def main():
         return run_batch(args)
     if args.workload == "cycle":
         return run_cycle(args)
+    if args.workload == "cd":
+        return run_cd(args)
 
     import p9idg
     from p9idg import observation
