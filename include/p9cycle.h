@@ -100,10 +100,13 @@ typedef struct p9cycle_stats {
  * x_image receives the model image float[G][G]; stats[max_major_cycles] (may
  * be NULL) and *ncycles the cycles run.  The deconvolver is FastSerialCD's
  * (the reference picks GPUSerialCD when a CUDA device exists: the same
- * iterations in batches of 1000 with racy ties).  FITS output is not written. */
-int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const void* vis, int32_t vis_type, const uint8_t* flags,
-                                int32_t psf_cut_factor, int32_t max_major_cycles, float lambda, float alpha,
-                                int32_t deconv_iterations, float deconv_epsilon, float* x_image,
+ * iterations in batches of 1000 with racy ties).  With obs_name set, every
+ * cycle writes <obs_name>_dirty_serial_majorCycle<k>.fits and
+ * <obs_name>_model_serial_majorCycle<k>.fits as the reference does
+ * (MajorCycle.cs:91,117; p9fits_write_image, float); NULL writes no files. */
+int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const char* obs_name, const void* vis, int32_t vis_type,
+                                const uint8_t* flags, int32_t psf_cut_factor, int32_t max_major_cycles, float lambda,
+                                float alpha, int32_t deconv_iterations, float deconv_epsilon, float* x_image,
                                 p9cycle_stats* stats, int32_t* ncycles);
 
 #ifdef __cplusplus

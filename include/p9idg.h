@@ -87,7 +87,8 @@ typedef enum p9idg_status {
     P9IDG_ERR_HIP = 6,
     P9IDG_ERR_ALLOC = 7,
     P9IDG_ERR_STATE = 8,
-    P9IDG_ERR_UNSUPPORTED = 9
+    P9IDG_ERR_UNSUPPORTED = 9,
+    P9IDG_ERR_IO = 10             /* a file that cannot be opened, read or parsed (p9fits.h) */
 } p9idg_status;
 
 typedef enum p9idg_vis_type { P9IDG_C128 = 0, P9IDG_C64 = 1 } p9idg_vis_type;

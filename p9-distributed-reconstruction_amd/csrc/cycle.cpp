@@ -9,10 +9,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "cd_internal.h"
 #include "p9cycle.h"
+#include "p9fits.h"
 #include "p9idg_internal.h"
 
 using namespace p9idg;
@@ -509,7 +511,8 @@ int device_max(const float* d, size_t n, float* d_out, float* out) {
 
 }  // namespace
 
-extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const void* vis, int32_t vis_type, const uint8_t* flags,
+extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const char* obs_name, const void* vis, int32_t vis_type,
+                                           const uint8_t* flags,
                                            int32_t psf_cut_factor, int32_t max_major_cycles, float lambda, float alpha,
                                            int32_t deconv_iterations, float deconv_epsilon, float* x_image,
                                            p9cycle_stats* stats, int32_t* ncycles) {
@@ -575,7 +578,17 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const void* vis, in
     auto ms = [](clock::time_point a, clock::time_point b) {
         return (float)std::chrono::duration<double, std::milli>(b - a).count();
     };
+    // FitsIO.Write of the dirty and model images of every cycle (MajorCycle.cs:91,117)
+    std::vector<float> host;
+    auto write_image = [&](const float* d_img, const char* what, int32_t cycle) -> int {
+        if (!obs_name) return P9IDG_OK;
+        host.resize(GG);
+        P9_HIP(hipMemcpy(host.data(), d_img, GG * sizeof(float), hipMemcpyDeviceToHost));
+        const std::string path = std::string(obs_name) + what + std::to_string(cycle) + ".fits";
+        return p9fits_write_image(path.c_str(), P9FITS_F32, host.data(), G, G);
+    };
     for (int32_t cycle = 0; cycle < max_major_cycles; cycle++) {
+        if (int st = write_image(d_image, "_dirty_serial_majorCycle", cycle)) return st;
         const auto t0 = clock::now();
         float max_dirty = 0.f, max_b = 0.f, abs_max = 0.f;
         if (int st = device_max(d_image, GG, b.max, &max_dirty)) return st;
@@ -594,6 +607,8 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const void* vis, in
                 return st;
         }
         const auto t2 = clock::now();
+        if (int st = write_image(b.x, "_model_serial_majorCycle", cycle)) return st;
+        const auto t3 = clock::now();
         if (stats) {
             p9cycle_stats& s = stats[cycle];
             s = p9cycle_stats{};
@@ -617,7 +632,7 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const void* vis, in
         P9_HIP(hipMemcpy(d_image, b.x, GG * sizeof(float), hipMemcpyDeviceToDevice));
         if (int st = p9idg_exec_cycle(plan, P9IDG_GRID_WSTACK)) return st;
         if (int st = p9idg_sync(plan)) return st;
-        if (stats) stats[cycle].imaging_ms = ms(t2, clock::now());
+        if (stats) stats[cycle].imaging_ms = ms(t3, clock::now());
     }
     P9_HIP(hipMemcpy(x_image, b.x, GG * sizeof(float), hipMemcpyDeviceToHost));
     return P9IDG_OK;

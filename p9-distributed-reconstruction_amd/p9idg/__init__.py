@@ -16,6 +16,7 @@ from .idg import (  # noqa: F401
     Visibilities,
 )
 from .cycle import CYCLE_STATS_DTYPE, PSF, DeconvolutionResult, FastSerialCD, MajorCycle, PaddedConvolver  # noqa: F401
+from .fitsio import FitsIO, MeasurementData  # noqa: F401
 from . import _lib  # noqa: F401
 from ._lib import (  # noqa: F401
     DEGRID_PLAIN,

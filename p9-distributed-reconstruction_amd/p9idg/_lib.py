@@ -24,6 +24,7 @@ ERR_HIP = 6
 ERR_ALLOC = 7
 ERR_STATE = 8
 ERR_UNSUPPORTED = 9
+ERR_IO = 10
 
 C128 = 0
 C64 = 1
@@ -170,9 +171,38 @@ CYCLE_SIGNATURES = {
     "p9cd_get_abs_max_diff_device": [_vp, _vp, _vp, _f32, _f32, _pf32],
     "p9cd_deconvolve": [_vp, _vp, _vp, _f32, _f32, _i32, _f32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     "p9cd_deconvolve_device": [_vp, _vp, _vp, _f32, _f32, _i32, _f32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
-    "p9idg_reconstruct_serial_cd": [_vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _i32, _f32, _vp, _vp,
-                                    ctypes.POINTER(_i32)],
+    "p9idg_reconstruct_serial_cd": [_vp, ctypes.c_char_p, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _i32, _f32, _vp,
+                                    _vp, ctypes.POINTER(_i32)],
 }
+
+# include/p9fits.h (host code of libp9idg.so: callable without a GPU)
+FITS_SIGNATURES = {
+    "p9fits_set_threads": [_i32],
+    "p9fits_read_info": [ctypes.c_char_p, _vp],
+    "p9fits_read_frequencies": [ctypes.c_char_p, _vp, _i64, _pi64],
+    "p9fits_count_baselines": [ctypes.c_char_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    "p9fits_read_uvw": [ctypes.c_char_p, _i32, _i32, _vp],
+    "p9fits_read_visibilities": [ctypes.c_char_p, _i32, _i32, _i32, _i32, _f64, _vp, _i32],
+    "p9fits_read_flags": [ctypes.c_char_p, _i32, _i32, _i32, _i32, _vp],
+    "p9fits_read_image": [ctypes.c_char_p, _i32, _vp, _i32, _i32],
+    "p9fits_read_casa": [ctypes.c_char_p, _vp, _i32, _i32],
+    "p9fits_write_image": [ctypes.c_char_p, _i32, _vp, _i32, _i32],
+}
+FITS_MAX_AXES = 8
+FITS_F32 = 0
+FITS_F64 = 1
+FITS_C128_REAL = 2
+FITS_C128_IMAG = 3
+
+
+class FitsInfo(ctypes.Structure):
+    _fields_ = [
+        ("bitpix", ctypes.c_int32),
+        ("naxis", ctypes.c_int32),
+        ("naxes", ctypes.c_int64 * FITS_MAX_AXES),
+        ("data_offset", ctypes.c_int64),
+        ("data_bytes", ctypes.c_int64),
+    ]
 _RESTYPES = {"p9idg_last_error": ctypes.c_char_p}
 
 OBS_SIGNATURES = {
@@ -198,7 +228,7 @@ def idg():
                 % (IDG_LIB_PATH, LIB_DIR)
             )
         lib = ctypes.CDLL(IDG_LIB_PATH)
-        for table in (IDG_SIGNATURES, CYCLE_SIGNATURES):
+        for table in (IDG_SIGNATURES, CYCLE_SIGNATURES, FITS_SIGNATURES):
             for name, args in table.items():
                 fn = getattr(lib, name)
                 fn.argtypes = args

@@ -199,17 +199,20 @@ class FastSerialCD:
 class MajorCycle:
     @staticmethod
     def ReconstructSerialCD(plan, visibilities, flags, psfCutFactor, maxMajorCycle, lambda_, alpha,
-                            deconvIterations, deconvEpsilon):
+                            deconvIterations, deconvEpsilon, obsName=None):
         """MajorCycle.ReconstructSerialCD (MajorCycle.cs:33-134) on a Plan
         whose observation is set; returns (xImage float32[G, G], stats
-        structured array, one row per cycle run).  FITS files are not written."""
+        structured array, one row per cycle run).  With obsName, every cycle
+        writes <obsName>_dirty_serial_majorCycle<k>.fits and
+        <obsName>_model_serial_majorCycle<k>.fits (MajorCycle.cs:91,117)."""
         vis, vtype = _vis_arg(visibilities)
         fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
         G = plan.c.GridSize
         x = np.zeros((G, G), dtype=np.float32)
         stats = np.zeros(max(int(maxMajorCycle), 1), dtype=CYCLE_STATS_DTYPE)
         n = ctypes.c_int32()
-        check(_lib.idg().p9idg_reconstruct_serial_cd(plan._h, ptr(vis), vtype, ptr(fl), int(psfCutFactor),
+        name = None if obsName is None else str(obsName).encode()
+        check(_lib.idg().p9idg_reconstruct_serial_cd(plan._h, name, ptr(vis), vtype, ptr(fl), int(psfCutFactor),
                                                      int(maxMajorCycle), float(lambda_), float(alpha),
                                                      int(deconvIterations), float(deconvEpsilon), ptr(x), ptr(stats),
                                                      ctypes.byref(n)))

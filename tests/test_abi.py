@@ -38,6 +38,14 @@ def test_cycle_functions_exported():
     assert set(names) == set(_lib.CYCLE_SIGNATURES)
 
 
+def test_fits_functions_exported():
+    lib = _lib.idg()
+    names = declared("p9fits.h", "p9fits_")
+    assert len(names) == 10
+    assert [n for n in names if not hasattr(lib, n)] == []
+    assert set(names) == set(_lib.FITS_SIGNATURES)
+
+
 def test_obs_library_exports_every_declared_function():
     lib = _lib.obs()
     names = declared("p9obs.h", "p9obs_")

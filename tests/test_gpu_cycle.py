@@ -83,7 +83,7 @@ def test_fast_serial_cd_matches_oracle_bit_for_bit(O, H, ph, lam, alpha, iters, 
 
 
 @pytest.mark.parametrize("kernel", ["fp32", "mfma_bf16"])
-def test_reconstruct_serial_cd_against_oracle(O, kernel):
+def test_reconstruct_serial_cd_against_oracle(O, kernel, tmp_path):
     cell = obs.ARCSEC * 20.0
     ob = obs.custom(G=128, S=16, K=4, L=4, n_ant=10, T=40, C=4, seed=95, core_diam=800.0, cell_arcsec=20.0,
                     sources=[(1.0, 9 * cell, -5 * cell, 0.0), (0.6, -20 * cell, 14 * cell, 0.0),
@@ -94,7 +94,7 @@ def test_reconstruct_serial_cd_against_oracle(O, kernel):
     xr, sr = O.reconstruct_serial_cd(oc, part.subgrids, ob.vis, None, ob.uvw, ob.freq, *args)
     with p9idg.Plan(ob.c, kernel=kernel) as plan:
         plan.set_observation(ob.uvw, ob.freq, part)
-        x, s = p9idg.MajorCycle.ReconstructSerialCD(plan, ob.vis, None, *args)
+        x, s = p9idg.MajorCycle.ReconstructSerialCD(plan, ob.vis, None, *args, obsName=str(tmp_path / "obs"))
     print(kernel, "cycles", len(s), len(sr))
     print("gpu", s[["max_dirty", "lambda_", "abs_max_diff", "iterations"]])
     print("ref", sr[["max_dirty", "lambda_", "abs_max_diff", "iterations"]])
@@ -106,6 +106,13 @@ def test_reconstruct_serial_cd_against_oracle(O, kernel):
     print("model image max rel err %.2e" % err)
     assert err <= 1e-5
     assert np.unravel_index(np.argmax(x), x.shape) == np.unravel_index(np.argmax(xr), xr.shape)
+    # the per-cycle FITS images (MajorCycle.cs:91,117)
+    for k in range(len(s)):
+        dirty = p9idg.FitsIO.ReadImage(str(tmp_path / ("obs_dirty_serial_majorCycle%d.fits" % k)))
+        model = p9idg.FitsIO.ReadImage(str(tmp_path / ("obs_model_serial_majorCycle%d.fits" % k)))
+        assert dirty.shape == model.shape == x.shape
+        assert max(dirty.max(), 0.0) == s["max_dirty"][k]
+    assert np.array_equal(model, x)
 
 
 def test_reconstruct_five_cycles_medium(O):
