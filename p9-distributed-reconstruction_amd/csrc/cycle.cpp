@@ -565,7 +565,8 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const char* obs_nam
     if (int st = alloc(&b.x, GG)) return st;
     if (int st = alloc(&b.grad, GG)) return st;
     if (int st = alloc(&b.max, 1)) return st;
-    P9_HIP(hipMemset(b.x, 0, GG * sizeof(float)));
+    P9_HIP(hipMemsetAsync(b.x, 0, GG * sizeof(float), nullptr));
+    P9_HIP(hipStreamSynchronize(nullptr));
     void* img_v = nullptr;
     if (int st = p9idg_device_buffers(plan, nullptr, nullptr, nullptr, nullptr, &img_v)) return st;
     float* d_image = static_cast<float*>(img_v);
@@ -629,7 +630,11 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const char* obs_nam
         }
         if (abs_max < kMajorEpsilon || cycle + 1 == max_major_cycles) break;
         // ToUVW, DeGridW, residual (MajorCycle.cs:125-129) and the next dirty image
-        P9_HIP(hipMemcpy(d_image, b.x, GG * sizeof(float), hipMemcpyDeviceToDevice));
+        // the plan's stream does not wait for the null stream, and a
+        // device-to-device hipMemcpy may return before the copy is done:
+        // finish it before ToUVW reads the image on the plan's stream
+        P9_HIP(hipMemcpyAsync(d_image, b.x, GG * sizeof(float), hipMemcpyDeviceToDevice, nullptr));
+        P9_HIP(hipStreamSynchronize(nullptr));
         if (int st = p9idg_exec_cycle(plan, P9IDG_GRID_WSTACK)) return st;
         if (int st = p9idg_sync(plan)) return st;
         if (stats) stats[cycle].imaging_ms = ms(t3, clock::now());

@@ -42,6 +42,16 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // half of dword q), so pairs are converted with one v_cvt_pk instruction.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// (a, b) = hi + lo per value, both rounded to nearest in the operand type.
+struct HiLo {
+    uint32_t hi, lo;
+};
+// Run-time -1 operands of the split's residual (KernelArgs::split_k).
+struct SplitK {
+    uint32_t bf16_lo, bf16_hi, f16;
+};
+__device__ __forceinline__ SplitK split_k(const KernelArgs& a) { return {a.split_k[0], a.split_k[1], a.split_k[2]}; }
+
 struct PrecBF16 {
     typedef __bf16 T;
     typedef bf16x8 V;
@@ -55,8 +65,17 @@ struct PrecBF16 {
     __device__ static __forceinline__ uint32_t pack(float a, float b) {
         return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
     }
-    __device__ static __forceinline__ f32x2 unpack(uint32_t h) {
-        return __builtin_convertvector(__builtin_bit_cast(bf16x2, h), f32x2);
+    // a - hi.lo and b - hi.hi, exact: hi (rounded to nearest) is within a
+    // factor of two of the value, so the difference is representable; each
+    // is one dot2 of hi with (-1, 0) or (0, -1) accumulated onto the value
+    __device__ static __forceinline__ HiLo split(float a, float b, SplitK k) {
+        HiLo r;
+        r.hi = pack(a, b);
+        const bf16x2 h = __builtin_bit_cast(bf16x2, r.hi);
+        const float la = __builtin_amdgcn_fdot2_f32_bf16(h, __builtin_bit_cast(bf16x2, k.bf16_lo), a, false);
+        const float lb = __builtin_amdgcn_fdot2_f32_bf16(h, __builtin_bit_cast(bf16x2, k.bf16_hi), b, false);
+        r.lo = pack(la, lb);
+        return r;
     }
 };
 struct PrecF16 {
@@ -72,27 +91,24 @@ struct PrecF16 {
     __device__ static __forceinline__ uint32_t pack(float a, float b) {
         return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, f16x2));
     }
-    __device__ static __forceinline__ f32x2 unpack(uint32_t h) {
-        return __builtin_convertvector(__builtin_bit_cast(f16x2, h), f32x2);
+    // a - hi.lo and b - hi.hi, exact (as for BF16), each one mixed-precision
+    // FMA of the FP16 half with -1
+    __device__ static __forceinline__ HiLo split(float a, float b, SplitK k) {
+        HiLo r;
+        r.hi = pack(a, b);
+        const f16x2 h = __builtin_bit_cast(f16x2, r.hi);
+        const float m1 = (float)__builtin_bit_cast(_Float16, (uint16_t)k.f16);
+        const float la = __builtin_fmaf((float)h.x, m1, a);
+        const float lb = __builtin_fmaf((float)h.y, m1, b);
+        r.lo = pack(la, lb);
+        return r;
     }
 };
 
-// (a, b) = hi + lo per value, both rounded to nearest in the operand type.
-struct HiLo {
-    uint32_t hi, lo;
-};
-template <class P>
-__device__ __forceinline__ HiLo split2(float a, float b) {
-    HiLo r;
-    r.hi = P::pack(a, b);
-    const f32x2 h = P::unpack(r.hi);
-    r.lo = P::pack(a - h.x, b - h.y);
-    return r;
-}
 // Writes the split of (a, b) into the operand dwords EH (hi) and EL (lo).
-#define P9_SPLIT2(P, a, b, EH, EL)               \
+#define P9_SPLIT2(P, K, a, b, EH, EL)            \
     do {                                         \
-        const HiLo s_ = split2<P>((a), (b));     \
+        const HiLo s_ = P::split((a), (b), (K)); \
         EH = s_.hi;                              \
         EL = s_.lo;                              \
     } while (0)
@@ -160,6 +176,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
+    const SplitK sk = split_k(a);
     const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
@@ -190,7 +207,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
     const double lcol = (double)compute_l(tx * 32 + li, S, a.image_size);
     const double ncol = axis_n(tx * 32 + li, S, a.image_size);
 
-    f32x16 acc0r = {}, acc0i = {}, acc1r = {}, acc1i = {};
+    f32x16 acc0r = {}, acc0i = {}, acc1r = {}, acc1i = {}, acc0n = {}, acc1n = {};
 
     // The next round's uvw, wavenumbers and visibilities are fetched into
     // registers while the current round is on the MFMA, then staged in LDS
@@ -252,12 +269,12 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
                             B = cmulf(B, RB);
                         }
                     }
-                    P9_SPLIT2(P, L[0].x, L[1].x, Lrh[q], Lrl[q]);
-                    P9_SPLIT2(P, L[0].y, L[1].y, Lih[q], Lil[q]);
-                    P9_SPLIT2(P, Aq[0].x, Aq[1].x, Arh[q], Arl[q]);
-                    P9_SPLIT2(P, Aq[0].y, Aq[1].y, Aih[q], Ail[q]);
                     cLr[q] = P::pack(cf[0] * L[0].x, cf[1] * L[1].x);
                     cLi[q] = P::pack(cf[0] * L[0].y, cf[1] * L[1].y);
+                    P9_SPLIT2(P, sk, L[0].x, L[1].x, Lrh[q], Lrl[q]);
+                    P9_SPLIT2(P, sk, L[0].y, L[1].y, Lih[q], Lil[q]);
+                    P9_SPLIT2(P, sk, Aq[0].x, Aq[1].x, Arh[q], Arl[q]);
+                    P9_SPLIT2(P, sk, Aq[0].y, Aq[1].y, Aih[q], Ail[q]);
                 }
             } else {
 #pragma unroll
@@ -272,31 +289,31 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
                         L[h] = cmulf(s_v[j], cis_f64(fma(cb, lrow, cc * nrow)));
                         cf[h] = (float)cc * cscale;
                     }
-                    P9_SPLIT2(P, L[0].x, L[1].x, Lrh[q], Lrl[q]);
-                    P9_SPLIT2(P, L[0].y, L[1].y, Lih[q], Lil[q]);
-                    P9_SPLIT2(P, A[0].x, A[1].x, Arh[q], Arl[q]);
-                    P9_SPLIT2(P, A[0].y, A[1].y, Aih[q], Ail[q]);
                     cLr[q] = P::pack(cf[0] * L[0].x, cf[1] * L[1].x);
                     cLi[q] = P::pack(cf[0] * L[0].y, cf[1] * L[1].y);
+                    P9_SPLIT2(P, sk, L[0].x, L[1].x, Lrh[q], Lrl[q]);
+                    P9_SPLIT2(P, sk, L[0].y, L[1].y, Lih[q], Lil[q]);
+                    P9_SPLIT2(P, sk, A[0].x, A[1].x, Arh[q], Arl[q]);
+                    P9_SPLIT2(P, sk, A[0].y, A[1].y, Aih[q], Ail[q]);
                 }
             }
-            const u32x4 nLih = neg(Lih), nLil = neg(Lil), ncLi = neg(cLi);
-            // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi)
+            // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi);
+            // the Li Ai products accumulate apart and are subtracted at the end
             acc0r = P::mma(Lrh, Arh, acc0r);
             acc0i = P::mma(Lrh, Aih, acc0i);
             acc1r = P::mma(cLr, Arh, acc1r);
             acc1i = P::mma(cLr, Aih, acc1i);
             acc0r = P::mma(Lrh, Arl, acc0r);
             acc0i = P::mma(Lrh, Ail, acc0i);
-            acc1r = P::mma(ncLi, Aih, acc1r);
+            acc1n = P::mma(cLi, Aih, acc1n);
             acc1i = P::mma(cLi, Arh, acc1i);
             acc0r = P::mma(Lrl, Arh, acc0r);
             acc0i = P::mma(Lrl, Aih, acc0i);
-            acc0r = P::mma(nLih, Aih, acc0r);
+            acc0n = P::mma(Lih, Aih, acc0n);
             acc0i = P::mma(Lih, Arh, acc0i);
-            acc0r = P::mma(nLih, Ail, acc0r);
+            acc0n = P::mma(Lih, Ail, acc0n);
             acc0i = P::mma(Lih, Arl, acc0i);
-            acc0r = P::mma(nLil, Aih, acc0r);
+            acc0n = P::mma(Lil, Aih, acc0n);
             acc0i = P::mma(Lil, Arh, acc0i);
         }
         __syncthreads();
@@ -313,8 +330,8 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
                 const int y = ty * 32 + 8 * (r / 4) + 4 * grp + (r % 4), x = tx * 32 + li;
                 if (y < S && x < S) {
                     const float rr = kTwoPiF * a.rtab[y * S + x];
-                    const float m1r = acc1r[r] * back1, m1i = acc1i[r] * back1;
-                    const float gr = acc0r[r] * back - rr * m1i;
+                    const float m1r = (acc1r[r] - acc1n[r]) * back1, m1i = acc1i[r] * back1;
+                    const float gr = (acc0r[r] - acc0n[r]) * back - rr * m1i;
                     const float gi = acc0i[r] * back + rr * m1r;
                     float2& d = s_tile[0][((y + S / 2) % S) * S + (x + S / 2) % S];
                     d.x += gr;
@@ -346,6 +363,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
+    const SplitK sk = split_k(a);
     const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, grp = lane >> 4;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
@@ -389,8 +407,8 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                     rr[u] = a.rtab[y * S + x] * rscale;
                 }
             }
-            P9_SPLIT2(P, pv[0].x, pv[1].x, Prh[rt][q], Prl[rt][q]);
-            P9_SPLIT2(P, pv[0].y, pv[1].y, Pih[rt][q], Pil[rt][q]);
+            P9_SPLIT2(P, sk, pv[0].x, pv[1].x, Prh[rt][q], Prl[rt][q]);
+            P9_SPLIT2(P, sk, pv[0].y, pv[1].y, Pih[rt][q], Pil[rt][q]);
             Qr[rt][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
             Qi[rt][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
@@ -405,8 +423,8 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
         u32x4 Arh, Arl, Aih, Ail;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            P9_SPLIT2(P, A[2 * q].x, A[2 * q + 1].x, Arh[q], Arl[q]);
-            P9_SPLIT2(P, A[2 * q].y, A[2 * q + 1].y, Aih[q], Ail[q]);
+            P9_SPLIT2(P, sk, A[2 * q].x, A[2 * q + 1].x, Arh[q], Arl[q]);
+            P9_SPLIT2(P, sk, A[2 * q].y, A[2 * q + 1].y, Aih[q], Ail[q]);
         }
         const u32x4 nAih = neg(Aih), nAil = neg(Ail);
         f32x4 q0r[RT], q0i[RT], q1r[RT], q1i[RT];
@@ -501,15 +519,26 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 // ch + 2 grp and ch + 2 grp + 1, so a timestep's channels are
                 // written by two back-to-back stores of whole cache lines
                 float2 keep0 = make_float2(0.f, 0.f), keep1 = keep0;
+                // two channels per trip (a tail channel beyond ne is computed
+                // and discarded), so the stepped phasors alternate between two
+                // register sets instead of being copied back every channel
 #pragma unroll 1
-                for (int e = 0; e < ne; e++) {
-                    const float2 vis = tile(A, B, fmaf((float)e, dcf, cf0));
-                    if (e == 2 * grp) keep0 = vis;
-                    if (e == 2 * grp + 1) keep1 = vis;
+                for (int e = 0; e < ne; e += 2) {
 #pragma unroll
-                    for (int i = 0; i < 8; i++) A[i] = cmulf(A[i], RA[i]);
+                    for (int h = 0; h < 2; h++) {
+                        // the next channel's x-phasors first: the split in
+                        // tile() may then overwrite this channel's in place
+                        float2 An[8];
 #pragma unroll
-                    for (int i = 0; i < NB; i++) B[i] = cmulf(B[i], RB[i]);
+                        for (int i = 0; i < 8; i++) An[i] = cmulf(A[i], RA[i]);
+                        const float2 vis = tile(A, B, fmaf((float)(e + h), dcf, cf0));
+                        if (e + h == 2 * grp) keep0 = vis;
+                        if (e + h == 2 * grp + 1) keep1 = vis;
+#pragma unroll
+                        for (int i = 0; i < 8; i++) A[i] = An[i];
+#pragma unroll
+                        for (int i = 0; i < NB; i++) B[i] = cmulf(B[i], RB[i]);
+                    }
                 }
                 const int j0 = tau * C + ch + 2 * grp;
                 if (2 * grp < ne && tau < thi && j0 >= k0 && j0 < K) a.vis_out[vbase + j0] = keep0;
@@ -552,6 +581,7 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
     __shared__ float s_red[4];
 
     const SubgridGeom g = load_geom(a, sidx, S);
+    const SplitK sk = split_k(a);
     const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, grp = lane >> 5;
     const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
@@ -595,8 +625,8 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
                     rr[u] = a.rtab[y * S + x] * rscale;
                 }
             }
-            P9_SPLIT2(P, pv[0].x, pv[1].x, prh[w], prl[w]);
-            P9_SPLIT2(P, pv[0].y, pv[1].y, pih[w], pil[w]);
+            P9_SPLIT2(P, sk, pv[0].x, pv[1].x, prh[w], prl[w]);
+            P9_SPLIT2(P, sk, pv[0].y, pv[1].y, pih[w], pil[w]);
             qr[w] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
             qi[w] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
@@ -635,8 +665,8 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
                     // padded columns (x >= S) meet zero pixels: keep them finite
                     A[u] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
                 }
-                P9_SPLIT2(P, A[0].x, A[1].x, Arh[h][q], Arl[h][q]);
-                P9_SPLIT2(P, A[0].y, A[1].y, Aih[h][q], Ail[h][q]);
+                P9_SPLIT2(P, sk, A[0].x, A[1].x, Arh[h][q], Arl[h][q]);
+                P9_SPLIT2(P, sk, A[0].y, A[1].y, Aih[h][q], Ail[h][q]);
             }
         }
         const float cf = (float)cc * q1_back;

@@ -86,6 +86,12 @@ struct KernelArgs {
     // each aligned run of 8 visibilities is one timestep's 8 consecutive
     // channels and the MFMA kernels step their phasors channel to channel
     int32_t chan_rec;
+    // -1.0 as BF16 in the low half, as BF16 in the high half, and as FP16 in
+    // the low half (0x0000BF80, 0xBF800000, 0x0000BC00): the MFMA kernels'
+    // hi/lo split takes its residual a - hi in one dot2 (BF16) or mixed FMA
+    // (FP16) against these; as compile-time constants they would be folded
+    // back into a conversion and a subtraction
+    uint32_t split_k[3];
     const WorkItem* items; // work queue (longest first)
     int32_t nitems;
     int32_t* counter;      // next item; zeroed by the launcher before each launch
