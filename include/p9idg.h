@@ -145,14 +145,16 @@ int p9idg_get_kernel(const p9idg_plan* plan, int32_t* kernel);
 /* Launch shape of the gridder / degridder: queue != 0 runs persistent
  * workgroups over a work queue whose items hold at most grid_chunk /
  * degrid_chunk visibilities of one subgrid (0 = whole subgrids), with
- * `workgroups` persistent workgroups (0 = CUs x occupancy); queue == 0
- * launches one workgroup per subgrid.  Defaults: 1, 2048, 2048, 0. */
+ * `workgroups` persistent workgroups in each direction (0 = CUs x
+ * occupancy); queue == 0 launches one workgroup per subgrid.  Defaults:
+ * 1, 2048, 2048, 0.  get_launch reports the gridder's workgroups (autotune
+ * may pick another count for the degridder). */
 int p9idg_set_launch(p9idg_plan* plan, int32_t queue, int32_t grid_chunk, int32_t degrid_chunk,
                      int32_t workgroups);
 int p9idg_get_launch(const p9idg_plan* plan, int32_t* queue, int32_t* grid_chunk,
                      int32_t* degrid_chunk, int32_t* workgroups);
-/* Chooses the launch shape (queue or not, visibilities per work item for
- * each direction, persistent workgroups) by timing candidates on the plan's
+/* Chooses the launch shape (queue or not, visibilities per work item and
+ * persistent workgroups for each direction) by timing candidates on the plan's
  * observation with HIP events (median of reps runs each), in the w-stacked
  * modes when the constants have more than one w-layer and the plain modes
  * otherwise; keeps the fastest and returns its gridder / degridder kernel

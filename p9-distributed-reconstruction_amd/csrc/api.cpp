@@ -102,7 +102,8 @@ struct p9idg_plan {
     int32_t use_queue = 1;
     int32_t grid_chunk = 2048;
     int32_t degrid_chunk = 2048;
-    int32_t workgroups = 0;
+    int32_t workgroups = 0;         // gridder's persistent workgroups (0: CUs x occupancy)
+    int32_t workgroups_degrid = 0;  // the degridder's (its workgroups are half the size)
 
     hipfftHandle fft_one = 0;
     bool have_fft_one = false;
@@ -358,10 +359,10 @@ int upload_items(p9idg_plan* p) {
     return P9IDG_OK;
 }
 
-LaunchCfg launch_cfg(const p9idg_plan* p) {
+LaunchCfg launch_cfg(const p9idg_plan* p, bool grid) {
     LaunchCfg cfg;
     cfg.queue = p->use_queue != 0;
-    cfg.workgroups = p->workgroups;
+    cfg.workgroups = grid ? p->workgroups : p->workgroups_degrid;
     return cfg;
 }
 
@@ -421,9 +422,9 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     if (int st = span_begin(p, 0)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
-        e = launch_grid_fp32(a, p->c.subgrid_size, launch_cfg(p), p->stream);
+        e = launch_grid_fp32(a, p->c.subgrid_size, launch_cfg(p, true), p->stream);
     else
-        e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p), p->stream);
+        e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p, true), p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
     if (int st = span_end(p)) return st;
     if (int st_ = record(p, 2)) return st_;
@@ -447,9 +448,9 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     if (int st = span_begin(p, 1)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32)
-        e = launch_degrid_fp32(a, p->c.subgrid_size, launch_cfg(p), p->stream);
+        e = launch_degrid_fp32(a, p->c.subgrid_size, launch_cfg(p, false), p->stream);
     else
-        e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p), p->stream);
+        e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p, false), p->stream);
     if (e != hipSuccess) return hip_fail(e, "degridder launch");
     if (int st = span_end(p)) return st;
     if (int st_ = record(p, 4)) return st_;
@@ -756,6 +757,7 @@ int p9idg_set_launch(p9idg_plan* p, int32_t queue, int32_t grid_chunk, int32_t d
     p->grid_chunk = grid_chunk;
     p->degrid_chunk = degrid_chunk;
     p->workgroups = workgroups;
+    p->workgroups_degrid = workgroups;
     if (p->have_obs) return upload_items(p);
     return P9IDG_OK;
 }
@@ -1152,13 +1154,15 @@ int p9idg_autotune(p9idg_plan* p, int32_t reps, float* grid_ms, float* degrid_ms
     for (int chunk : {512, 1024, 2048, 4096, 0})
         for (int wg : {0, cus, 2 * cus, 4 * cus}) cands.push_back({1, chunk, wg});
     float best[2] = {1e30f, 1e30f};
-    Cand pick[2] = {{p->use_queue, p->grid_chunk, p->workgroups}, {p->use_queue, p->degrid_chunk, p->workgroups}};
+    Cand pick[2] = {{p->use_queue, p->grid_chunk, p->workgroups},
+                    {p->use_queue, p->degrid_chunk, p->workgroups_degrid}};
     const int32_t q0 = p->use_queue, g0 = p->grid_chunk, d0 = p->degrid_chunk, w0 = p->workgroups;
     for (const Cand& c : cands) {
         p->use_queue = c.queue;
         p->grid_chunk = c.chunk;
         p->degrid_chunk = c.chunk;
         p->workgroups = c.wg;
+        p->workgroups_degrid = c.wg;
         if (int st = upload_items(p)) return st;
         for (int dir = 0; dir < 2; dir++) {
             float ms = 0.f;
@@ -1176,11 +1180,13 @@ int p9idg_autotune(p9idg_plan* p, int32_t reps, float* grid_ms, float* degrid_ms
     (void)g0;
     (void)d0;
     (void)w0;
-    // the queue flag and workgroup count are shared by both directions: take
-    // them from the direction with the larger time, chunks from each
+    // the queue flag is shared by both directions: take it from the direction
+    // with the larger time, workgroups and chunks from each (with the queue
+    // off they are unused)
     const int dom = best[0] >= best[1] ? 0 : 1;
     p->use_queue = pick[dom].queue;
-    p->workgroups = pick[dom].wg;
+    p->workgroups = pick[0].wg;
+    p->workgroups_degrid = pick[1].wg;
     p->grid_chunk = pick[0].chunk;
     p->degrid_chunk = pick[1].chunk;
     p->collect = collect;
@@ -1334,7 +1340,7 @@ struct p9idg_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     struct Group {
-        int S = 0, kernel = 0, workgroups = 0;
+        int S = 0, kernel = 0, workgroups = 0, workgroups_degrid = 0;
         std::vector<int> members;  // plan indices; item.inst indexes this list
         KernelArgs* d_args_grid = nullptr;
         KernelArgs* d_args_degrid = nullptr;
@@ -1394,7 +1400,7 @@ int batch_run(p9idg_batch* b, int what, bool capturing) {
         if (!capturing) P9_HIP(hipEventRecord(b->ev[3], b->stream));
         for (auto& g : b->groups) {
             BatchArgs ba{g.d_args_degrid, g.d_items_degrid, g.n_degrid, g.d_counter};
-            hipError_t e = launch_degrid_batch(ba, g.S, g.kernel, g.workgroups, b->stream);
+            hipError_t e = launch_degrid_batch(ba, g.S, g.kernel, g.workgroups_degrid, b->stream);
             if (e != hipSuccess) return hip_fail(e, "batched degridder launch");
         }
     }
@@ -1469,6 +1475,7 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
             g.S = p->c.subgrid_size;
             g.kernel = p->kernel;
             g.workgroups = p->workgroups;
+            g.workgroups_degrid = p->workgroups_degrid;
             g.members.push_back(i);
             b->groups.push_back(g);
         }
