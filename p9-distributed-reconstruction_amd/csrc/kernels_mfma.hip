@@ -358,9 +358,13 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     static_assert(S <= 32, "x fits one 32-deep k step");
     constexpr int RT = (S + 15) / 16;  // row tiles
     constexpr int NB = 4 * RT;         // y-phasors per lane
+    constexpr int NR = (8 + NB) / 2;   // float4 slots of a lane's channel-step phasors
     __shared__ float2 s_tile[2][S * S];
     __shared__ double s_l[32], s_n1[32];
     __shared__ float s_red[4];
+    // each lane's channel-step phasors (RA, then RB, two per float4), kept in
+    // LDS rather than in 2 (8 + NB) registers: read once per channel
+    __shared__ float4 s_rstep[NR][kThreads];
 
     const SubgridGeom g = load_geom(a, sidx, S);
     const SplitK sk = split_k(a);
@@ -508,11 +512,22 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 const double k = a.wavenumbers[ch];
                 const double dk = a.wavenumbers[ch + 1 < C ? ch + 1 : ch] - k;
                 const double cc = (g.w_off - k * w) * kInvTwoPi, dc = -dk * w * kInvTwoPi;
-                float2 A[8], RA[8], B[NB], RB[NB];
+                float2 A[8], B[NB];
                 xphasors((g.u_off - k * u) * kInvTwoPi, cc, A);
-                xphasors(-dk * u * kInvTwoPi, dc, RA);
                 yphasors((g.v_off - k * v) * kInvTwoPi, cc, B);
-                yphasors(-dk * v * kInvTwoPi, dc, RB);
+                {
+                    float2 RA[8], RB[NB];
+                    xphasors(-dk * u * kInvTwoPi, dc, RA);
+                    yphasors(-dk * v * kInvTwoPi, dc, RB);
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        s_rstep[i][tid] = make_float4(RA[2 * i].x, RA[2 * i].y, RA[2 * i + 1].x, RA[2 * i + 1].y);
+#pragma unroll
+                    for (int i = 0; i < NB / 2; i++)
+                        s_rstep[4 + i][tid] = make_float4(RB[2 * i].x, RB[2 * i].y, RB[2 * i + 1].x, RB[2 * i + 1].y);
+                    // reloaded from LDS below, not forwarded in registers
+                    asm volatile("" ::: "memory");
+                }
                 const float cf0 = (float)cc * q1_back, dcf = (float)dc * q1_back;
                 const int ne = min(8, run - cb);
                 // every lane holds the sums; lane group grp keeps channels
@@ -530,14 +545,22 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                         // tile() may then overwrite this channel's in place
                         float2 An[8];
 #pragma unroll
-                        for (int i = 0; i < 8; i++) An[i] = cmulf(A[i], RA[i]);
+                        for (int i = 0; i < 4; i++) {
+                            const float4 r = s_rstep[i][tid];
+                            An[2 * i] = cmulf(A[2 * i], make_float2(r.x, r.y));
+                            An[2 * i + 1] = cmulf(A[2 * i + 1], make_float2(r.z, r.w));
+                        }
                         const float2 vis = tile(A, B, fmaf((float)(e + h), dcf, cf0));
                         if (e + h == 2 * grp) keep0 = vis;
                         if (e + h == 2 * grp + 1) keep1 = vis;
 #pragma unroll
                         for (int i = 0; i < 8; i++) A[i] = An[i];
 #pragma unroll
-                        for (int i = 0; i < NB; i++) B[i] = cmulf(B[i], RB[i]);
+                        for (int i = 0; i < NB / 2; i++) {
+                            const float4 r = s_rstep[4 + i][tid];
+                            B[2 * i] = cmulf(B[2 * i], make_float2(r.x, r.y));
+                            B[2 * i + 1] = cmulf(B[2 * i + 1], make_float2(r.z, r.w));
+                        }
                     }
                 }
                 const int j0 = tau * C + ch + 2 * grp;
