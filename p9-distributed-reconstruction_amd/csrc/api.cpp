@@ -390,6 +390,7 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
     a.degrid_scale = 1.0f / (float)(S * S);                      // Gridder.cs:151
     a.mode = mode;
     a.chan_rec = p->chan_uniform && p->C % 8 == 0;
+    a.chan_step = p->chan_uniform;
     a.split_k[0] = 0x0000BF80u;
     a.split_k[1] = 0xBF800000u;
     a.split_k[2] = 0x0000BC00u;

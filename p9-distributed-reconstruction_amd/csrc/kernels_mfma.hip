@@ -346,7 +346,200 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 }
 
 // ---------------------------------------------------------------------------
-// Degridder for S <= 32 on 16x16x32 MFMA tiles.  One workgroup per work item:
+// Gridder for S = 48 on 16x16x32 MFMA tiles: the subgrid is 3 x 3 tiles of
+// 16 x 16 (no padding to 64).  One 256-thread workgroup per work item; a step
+// is 32 visibilities, the 4 waves take alternate steps and each accumulates
+// all 9 tiles.  Lane l = 16 g + c: c is the row (A operand) and column (B
+// operand) within a tile, g picks the lane's 8 visibilities of the step.
+// Per step the x-phasors of the 3 column tiles are built once, then each row
+// tile's V * y-phasor operands are built and multiplied into its 3 tiles.
+template <int S, class P>
+__device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, int k0, int k1) {
+    constexpr int NT = S / 16;  // tiles per axis
+    static_assert(NT * 16 == S, "subgrid size a multiple of 16");
+    constexpr int PER = 2;  // visibilities staged per thread and round
+    constexpr int ROUND = PER * kThreads;
+    __shared__ double s_a[ROUND], s_b[ROUND], s_c[ROUND];
+    __shared__ float2 s_v[ROUND];
+    __shared__ float2 s_tile[2][S * S];
+    __shared__ float s_red[4];
+
+    const SubgridGeom g = load_geom(a, sidx, S);
+    const SplitK sk = split_k(a);
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 15, grp = lane >> 4;
+    const int C = a.C, K = k1 < 0 ? g.nt * C : k1;
+    const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
+    const bool chan_rec = a.chan_rec != 0;
+
+    for (int p = tid; p < S * S; p += kThreads) s_tile[0][p] = make_float2(0.f, 0.f);
+    __syncthreads();
+
+    // FP16: scale the visibilities so |V| <= 1, and c * L by 2^-16.
+    float vscale = 1.f;
+    float cscale = 1.f;
+    if (P::kNormalise) {
+        float m = 0.f;
+        for (int j = k0 + tid; j < K; j += kThreads) {
+            const float2 v = a.vis[vbase + j];
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+        vscale = 1.f / pow2_ceil(block_max(m, s_red));
+        cscale = 1.f / 65536.f;
+    }
+
+    // l and the separable n of coordinate 16 t + c (rows and columns alike)
+    double lt[NT], nt[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        lt[t] = (double)compute_l(16 * t + c, S, a.image_size);
+        nt[t] = axis_n(16 * t + c, S, a.image_size);
+    }
+
+    f32x4 acc0r[NT][NT], acc0i[NT][NT], acc1r[NT][NT], acc1i[NT][NT];
+#pragma unroll
+    for (int r = 0; r < NT; r++)
+#pragma unroll
+        for (int q = 0; q < NT; q++) acc0r[r][q] = acc0i[r][q] = acc1r[r][q] = acc1i[r][q] = (f32x4){};
+
+    double fu[PER], fv[PER], fw[PER], fk[PER];
+    float2 fvis[PER];
+    auto fetch = [&](int j0) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int jc = min(j0 + i * kThreads + tid, K - 1);
+            const int t = g.t0 + jc / C, ch = jc % C;
+            const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
+            fu[i] = p[0];
+            fv[i] = p[1];
+            fw[i] = p[2];
+            fk[i] = a.wavenumbers[ch];
+            fvis[i] = a.vis[vbase + jc];
+        }
+    };
+    // the 8 phasors of this lane group's visibilities at coefficient pair
+    // (ca, cc) for coordinate tile t: channel stepping or direct evaluation
+    auto phasors = [&](int jg, const double* co, int t, float2 (&out)[8]) {
+        if (chan_rec) {
+            const double ca = co[jg], cc = s_c[jg];
+            const double da = co[jg + 1] - ca, dc = s_c[jg + 1] - cc;
+            float2 A = cis_f64(fma(ca, lt[t], cc * nt[t]));
+            const float2 R = cis_f64(fma(da, lt[t], dc * nt[t]));
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                out[e] = A;
+                if (e < 7) A = cmulf(A, R);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) out[e] = cis_f64(fma(co[jg + e], lt[t], s_c[jg + e] * nt[t]));
+        }
+    };
+
+    if (k0 < K) fetch(k0);
+    for (int j0 = k0; j0 < K; j0 += ROUND) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int idx = i * kThreads + tid;
+            const bool in = j0 + idx < K;
+            s_a[idx] = in ? (g.u_off - fk[i] * fu[i]) * kInvTwoPi : 0.0;
+            s_b[idx] = in ? (g.v_off - fk[i] * fv[i]) * kInvTwoPi : 0.0;
+            s_c[idx] = in ? (g.w_off - fk[i] * fw[i]) * kInvTwoPi : 0.0;
+            s_v[idx] = in ? make_float2(fvis[i].x * vscale, fvis[i].y * vscale) : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        if (j0 + ROUND < K) fetch(j0 + ROUND);
+        const int nsteps = (min(ROUND, K - j0) + 31) / 32;
+        for (int st = wave; st < nsteps; st += 4) {
+            const int jg = st * 32 + grp * 8;  // this lane group's 8 visibilities
+            // x-phasors of the column tiles, split
+            u32x4 Arh[NT], Arl[NT], Aih[NT], Ail[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                float2 A[8];
+                phasors(jg, s_a, t, A);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    P9_SPLIT2(P, sk, A[2 * q].x, A[2 * q + 1].x, Arh[t][q], Arl[t][q]);
+                    P9_SPLIT2(P, sk, A[2 * q].y, A[2 * q + 1].y, Aih[t][q], Ail[t][q]);
+                }
+            }
+            // c of the 8 visibilities (FP32; scaled for FP16)
+            float cf[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) cf[e] = (float)s_c[jg + e] * cscale;
+#pragma unroll
+            for (int r = 0; r < NT; r++) {
+                float2 B[8];
+                phasors(jg, s_b, r, B);
+                u32x4 Lrh, Lrl, Lih, Lil, cLr, cLi;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float2 L0 = cmulf(s_v[jg + 2 * q], B[2 * q]), L1 = cmulf(s_v[jg + 2 * q + 1], B[2 * q + 1]);
+                    cLr[q] = P::pack(cf[2 * q] * L0.x, cf[2 * q + 1] * L1.x);
+                    cLi[q] = P::pack(cf[2 * q] * L0.y, cf[2 * q + 1] * L1.y);
+                    P9_SPLIT2(P, sk, L0.x, L1.x, Lrh[q], Lrl[q]);
+                    P9_SPLIT2(P, sk, L0.y, L1.y, Lih[q], Lil[q]);
+                }
+                const u32x4 nLih = neg(Lih), nLil = neg(Lil), ncLi = neg(cLi);
+#pragma unroll
+                for (int q = 0; q < NT; q++) {
+                    // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi)
+                    f32x4& m0r = acc0r[r][q];
+                    f32x4& m0i = acc0i[r][q];
+                    m0r = P::mma16(Lrh, Arh[q], m0r);
+                    m0i = P::mma16(Lrh, Aih[q], m0i);
+                    acc1r[r][q] = P::mma16(cLr, Arh[q], acc1r[r][q]);
+                    acc1i[r][q] = P::mma16(cLr, Aih[q], acc1i[r][q]);
+                    m0r = P::mma16(Lrh, Arl[q], m0r);
+                    m0i = P::mma16(Lrh, Ail[q], m0i);
+                    acc1r[r][q] = P::mma16(ncLi, Aih[q], acc1r[r][q]);
+                    acc1i[r][q] = P::mma16(cLi, Arh[q], acc1i[r][q]);
+                    m0r = P::mma16(Lrl, Arh[q], m0r);
+                    m0i = P::mma16(Lrl, Aih[q], m0i);
+                    m0r = P::mma16(nLih, Aih[q], m0r);
+                    m0i = P::mma16(Lih, Arh[q], m0i);
+                    m0r = P::mma16(nLih, Ail[q], m0r);
+                    m0i = P::mma16(Lih, Arl[q], m0i);
+                    m0r = P::mma16(nLil, Aih[q], m0r);
+                    m0i = P::mma16(Lil, Arh[q], m0i);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // subgrid = (M0 + 2 pi i r M1) / vscale, summed over the 4 waves into the
+    // ifftshift-ed tile (Gridder.cs:76-79; identity taper).  Lane (c, g) holds
+    // rows 4 g + e of every tile at column c.
+    const float back = 1.f / vscale;
+    const float back1 = back / cscale;
+    for (int w = 0; w < 4; w++) {
+        if (wave == w) {
+#pragma unroll
+            for (int r = 0; r < NT; r++)
+#pragma unroll
+                for (int q = 0; q < NT; q++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int y = 16 * r + 4 * grp + e, x = 16 * q + c;
+                        const float rr = kTwoPiF * a.rtab[y * S + x];
+                        const float m1r = acc1r[r][q][e] * back1, m1i = acc1i[r][q][e] * back1;
+                        const float gr = acc0r[r][q][e] * back - rr * m1i;
+                        const float gi = acc0i[r][q][e] * back + rr * m1r;
+                        float2& d = s_tile[0][((y + S / 2) % S) * S + (x + S / 2) % S];
+                        d.x += gr;
+                        d.y += gi;
+                    }
+        }
+        __syncthreads();
+    }
+    const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, +1);
+    adder<S>(a, g, res, 1.0f / (float)(S * S));
+}
+
+// ---------------------------------------------------------------------------
+// Degridder for S <= 48 on 16x16x32 MFMA tiles.  One workgroup per work item:
 // splitter + forward FFT into LDS, then the subgrid pixels, split hi/lo, stay
 // in registers as the MFMA A operand (row tiles of 16 y, k = x padded to 32).
 // A tile is 16 visibilities (the N dimension): lane l holds the x-phasors
@@ -355,12 +548,14 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 // 4 RT y-phasors, small enough for two waves per SIMD.
 template <int S, class P>
 __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
-    static_assert(S <= 32, "x fits one 32-deep k step");
+    static_assert(S <= 48, "x in at most two 32-deep k steps, up to 3 row tiles");
     constexpr int RT = (S + 15) / 16;  // row tiles
+    constexpr int KS = (S + 31) / 32;  // k steps of 32 along x
+    constexpr int NX = 8 * KS;         // x-phasors per lane
     constexpr int NB = 4 * RT;         // y-phasors per lane
-    constexpr int NR = (8 + NB) / 2;   // float4 slots of a lane's channel-step phasors
+    constexpr int NR = (NX + NB) / 2;  // float4 slots of a lane's channel-step phasors
     __shared__ float2 s_tile[2][S * S];
-    __shared__ double s_l[32], s_n1[32];
+    __shared__ double s_l[32 * KS], s_n1[32 * KS];
     __shared__ float s_red[4];
     // each lane's channel-step phasors (RA, then RB, two per float4), kept in
     // LDS rather than in 2 (8 + NB) registers: read once per channel
@@ -374,7 +569,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
 
     splitter<S>(a, g, s_tile[0]);
-    if (tid < 32) {
+    if (tid < 32 * KS) {
         s_l[tid] = (double)compute_l(tid, S, a.image_size);
         s_n1[tid] = axis_n(tid, S, a.image_size);
     }
@@ -391,30 +586,33 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     const float rscale = 1.f / pow2_ceil(fabsf(a.rtab[0]));
 
     // A operand: pixels_copy[y][x] = subgrid[(y+S/2)%S][(x+S/2)%S] (Gridder.cs:101-117)
-    // at y = 16 rt + col, x = 8 grp + e; P split hi/lo and r P
-    u32x4 Prh[RT], Prl[RT], Pih[RT], Pil[RT], Qr[RT], Qi[RT];
+    // at y = 16 rt + col, x = 32 ks + 8 grp + e; P split hi/lo and r P
+    u32x4 Prh[RT][KS], Prl[RT][KS], Pih[RT][KS], Pil[RT][KS], Qr[RT][KS], Qi[RT][KS];
 #pragma unroll
     for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
-            float2 pv[2];
-            float rr[2];
+        for (int ks = 0; ks < KS; ks++) {
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const int y = 16 * rt + col, x = 8 * grp + 2 * q + u;
-                pv[u] = make_float2(0.f, 0.f);
-                rr[u] = 0.f;
-                if (y < S && x < S) {
-                    pv[u] = res[((y + S / 2) % S) * S + (x + S / 2) % S];
-                    pv[u].x *= pscale;
-                    pv[u].y *= pscale;
-                    rr[u] = a.rtab[y * S + x] * rscale;
+            for (int q = 0; q < 4; q++) {  // elements 2q, 2q+1
+                float2 pv[2];
+                float rr[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int y = 16 * rt + col, x = 32 * ks + 8 * grp + 2 * q + u;
+                    pv[u] = make_float2(0.f, 0.f);
+                    rr[u] = 0.f;
+                    if (y < S && x < S) {
+                        pv[u] = res[((y + S / 2) % S) * S + (x + S / 2) % S];
+                        pv[u].x *= pscale;
+                        pv[u].y *= pscale;
+                        rr[u] = a.rtab[y * S + x] * rscale;
+                    }
                 }
+                P9_SPLIT2(P, sk, pv[0].x, pv[1].x, Prh[rt][ks][q], Prl[rt][ks][q]);
+                P9_SPLIT2(P, sk, pv[0].y, pv[1].y, Pih[rt][ks][q], Pil[rt][ks][q]);
+                Qr[rt][ks][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
+                Qi[rt][ks][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
             }
-            P9_SPLIT2(P, sk, pv[0].x, pv[1].x, Prh[rt][q], Prl[rt][q]);
-            P9_SPLIT2(P, sk, pv[0].y, pv[1].y, Pih[rt][q], Pil[rt][q]);
-            Qr[rt][q] = P::pack(rr[0] * pv[0].x, rr[1] * pv[1].x);
-            Qi[rt][q] = P::pack(rr[0] * pv[0].y, rr[1] * pv[1].y);
         }
     }
     const float out_scale = a.degrid_scale / pscale;
@@ -422,36 +620,40 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
 
     // One tile: Q0 = P conj(A), Q1 = (r P) conj(A) on the MFMA, then
     // vis = sum_y conj(B_v[y]) (Q0 - 2 pi i c Q1) over this lane's rows and
-    // the four lane groups.  A: the 8 x-phasors, B: the NB y-phasors.
-    auto tile = [&](const float2 (&A)[8], const float2 (&B)[NB], float cf) -> float2 {
-        u32x4 Arh, Arl, Aih, Ail;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            P9_SPLIT2(P, sk, A[2 * q].x, A[2 * q + 1].x, Arh[q], Arl[q]);
-            P9_SPLIT2(P, sk, A[2 * q].y, A[2 * q + 1].y, Aih[q], Ail[q]);
-        }
-        const u32x4 nAih = neg(Aih), nAil = neg(Ail);
+    // the four lane groups.  A: the NX x-phasors, B: the NB y-phasors.
+    auto tile = [&](const float2 (&A)[NX], const float2 (&B)[NB], float cf) -> float2 {
         f32x4 q0r[RT], q0i[RT], q1r[RT], q1i[RT];
 #pragma unroll
-        for (int rt = 0; rt < RT; rt++) {
-            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai) (hi*hi + hi*lo + lo*hi)
-            f32x4 z = {};
-            q0r[rt] = P::mma16(Prh[rt], Arh, z);
-            q0i[rt] = P::mma16(Pih[rt], Arh, z);
-            q1r[rt] = P::mma16(Qr[rt], Arh, z);
-            q1i[rt] = P::mma16(Qi[rt], Arh, z);
-            q0r[rt] = P::mma16(Prh[rt], Arl, q0r[rt]);
-            q0i[rt] = P::mma16(Pih[rt], Arl, q0i[rt]);
-            q1r[rt] = P::mma16(Qi[rt], Aih, q1r[rt]);
-            q1i[rt] = P::mma16(Qr[rt], nAih, q1i[rt]);
-            q0r[rt] = P::mma16(Prl[rt], Arh, q0r[rt]);
-            q0i[rt] = P::mma16(Pil[rt], Arh, q0i[rt]);
-            q0r[rt] = P::mma16(Pih[rt], Aih, q0r[rt]);
-            q0i[rt] = P::mma16(Prh[rt], nAih, q0i[rt]);
-            q0r[rt] = P::mma16(Pih[rt], Ail, q0r[rt]);
-            q0i[rt] = P::mma16(Prh[rt], nAil, q0i[rt]);
-            q0r[rt] = P::mma16(Pil[rt], Aih, q0r[rt]);
-            q0i[rt] = P::mma16(Prl[rt], nAih, q0i[rt]);
+        for (int rt = 0; rt < RT; rt++) q0r[rt] = q0i[rt] = q1r[rt] = q1i[rt] = (f32x4){};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            u32x4 Arh, Arl, Aih, Ail;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                P9_SPLIT2(P, sk, A[8 * ks + 2 * q].x, A[8 * ks + 2 * q + 1].x, Arh[q], Arl[q]);
+                P9_SPLIT2(P, sk, A[8 * ks + 2 * q].y, A[8 * ks + 2 * q + 1].y, Aih[q], Ail[q]);
+            }
+            const u32x4 nAih = neg(Aih), nAil = neg(Ail);
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) {
+                // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai) (hi*hi + hi*lo + lo*hi)
+                q0r[rt] = P::mma16(Prh[rt][ks], Arh, q0r[rt]);
+                q0i[rt] = P::mma16(Pih[rt][ks], Arh, q0i[rt]);
+                q1r[rt] = P::mma16(Qr[rt][ks], Arh, q1r[rt]);
+                q1i[rt] = P::mma16(Qi[rt][ks], Arh, q1i[rt]);
+                q0r[rt] = P::mma16(Prh[rt][ks], Arl, q0r[rt]);
+                q0i[rt] = P::mma16(Pih[rt][ks], Arl, q0i[rt]);
+                q1r[rt] = P::mma16(Qi[rt][ks], Aih, q1r[rt]);
+                q1i[rt] = P::mma16(Qr[rt][ks], nAih, q1i[rt]);
+                q0r[rt] = P::mma16(Prl[rt][ks], Arh, q0r[rt]);
+                q0i[rt] = P::mma16(Pil[rt][ks], Arh, q0i[rt]);
+                q0r[rt] = P::mma16(Pih[rt][ks], Aih, q0r[rt]);
+                q0i[rt] = P::mma16(Prh[rt][ks], nAih, q0i[rt]);
+                q0r[rt] = P::mma16(Pih[rt][ks], Ail, q0r[rt]);
+                q0i[rt] = P::mma16(Prh[rt][ks], nAil, q0i[rt]);
+                q0r[rt] = P::mma16(Pil[rt][ks], Aih, q0r[rt]);
+                q0i[rt] = P::mma16(Prl[rt][ks], nAih, q0i[rt]);
+            }
         }
         float sr = 0.f, si = 0.f;
 #pragma unroll
@@ -473,10 +675,10 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     };
     // x of phasor e and y of phasor i for this lane; phasors outside the
     // subgrid meet zero pixels and are kept at 0 (finite)
-    auto xphasors = [&](double ca, double cc, float2 (&A)[8]) {
+    auto xphasors = [&](double ca, double cc, float2 (&A)[NX]) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int x = 8 * grp + e;
+        for (int e = 0; e < NX; e++) {
+            const int x = 32 * (e / 8) + 8 * grp + e % 8;
             A[e] = x < S ? cis_f64(fma(ca, s_l[x], cc * s_n1[x])) : make_float2(0.f, 0.f);
         }
     };
@@ -488,7 +690,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
         }
     };
 
-    if (a.chan_rec) {
+    if (a.chan_step) {
         // Channel recurrence.  Column col of every tile is one timestep and
         // consecutive tiles are consecutive channels, so each lane steps its
         // phasors by one multiplication per tile.  A super-tile is 16
@@ -512,19 +714,19 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 const double k = a.wavenumbers[ch];
                 const double dk = a.wavenumbers[ch + 1 < C ? ch + 1 : ch] - k;
                 const double cc = (g.w_off - k * w) * kInvTwoPi, dc = -dk * w * kInvTwoPi;
-                float2 A[8], B[NB];
+                float2 A[NX], B[NB];
                 xphasors((g.u_off - k * u) * kInvTwoPi, cc, A);
                 yphasors((g.v_off - k * v) * kInvTwoPi, cc, B);
                 {
-                    float2 RA[8], RB[NB];
+                    float2 RA[NX], RB[NB];
                     xphasors(-dk * u * kInvTwoPi, dc, RA);
                     yphasors(-dk * v * kInvTwoPi, dc, RB);
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
+                    for (int i = 0; i < NX / 2; i++)
                         s_rstep[i][tid] = make_float4(RA[2 * i].x, RA[2 * i].y, RA[2 * i + 1].x, RA[2 * i + 1].y);
 #pragma unroll
                     for (int i = 0; i < NB / 2; i++)
-                        s_rstep[4 + i][tid] = make_float4(RB[2 * i].x, RB[2 * i].y, RB[2 * i + 1].x, RB[2 * i + 1].y);
+                        s_rstep[NX / 2 + i][tid] = make_float4(RB[2 * i].x, RB[2 * i].y, RB[2 * i + 1].x, RB[2 * i + 1].y);
                     // reloaded from LDS below, not forwarded in registers
                     asm volatile("" ::: "memory");
                 }
@@ -543,9 +745,9 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                     for (int h = 0; h < 2; h++) {
                         // the next channel's x-phasors first: the split in
                         // tile() may then overwrite this channel's in place
-                        float2 An[8];
+                        float2 An[NX];
 #pragma unroll
-                        for (int i = 0; i < 4; i++) {
+                        for (int i = 0; i < NX / 2; i++) {
                             const float4 r = s_rstep[i][tid];
                             An[2 * i] = cmulf(A[2 * i], make_float2(r.x, r.y));
                             An[2 * i + 1] = cmulf(A[2 * i + 1], make_float2(r.z, r.w));
@@ -554,10 +756,10 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                         if (e + h == 2 * grp) keep0 = vis;
                         if (e + h == 2 * grp + 1) keep1 = vis;
 #pragma unroll
-                        for (int i = 0; i < 8; i++) A[i] = An[i];
+                        for (int i = 0; i < NX; i++) A[i] = An[i];
 #pragma unroll
                         for (int i = 0; i < NB / 2; i++) {
-                            const float4 r = s_rstep[4 + i][tid];
+                            const float4 r = s_rstep[NX / 2 + i][tid];
                             B[2 * i] = cmulf(B[2 * i], make_float2(r.x, r.y));
                             B[2 * i + 1] = cmulf(B[2 * i + 1], make_float2(r.z, r.w));
                         }
@@ -581,7 +783,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
         const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
         const double k = a.wavenumbers[ch];
         const double cc = (g.w_off - k * p[2]) * kInvTwoPi;
-        float2 A[8], B[NB];
+        float2 A[NX], B[NB];
         xphasors((g.u_off - k * p[0]) * kInvTwoPi, cc, A);
         yphasors((g.v_off - k * p[1]) * kInvTwoPi, cc, B);
         const float2 vis = tile(A, B, (float)cc * q1_back);
@@ -590,7 +792,7 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
 }
 
 
-// Degridder for S = 48, 64: rows and x span two 32-wide tiles.  The split
+// Degridder for S = 64: rows and x span two 32-wide tiles.  The split
 // pixel operands (2 row tiles x 4 k-steps of 16) live in LDS, pre-arranged
 // in MFMA A-operand lane order; each wave takes tiles of 32 visibilities and
 // accumulates both row tiles in turn.
@@ -741,22 +943,31 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
 
 // two waves per SIMD up to S = 48; the S = 64 gridder needs more registers
 template <int S>
-constexpr int kGridWaves = S <= 48 ? 2 : 1;
+constexpr int kGridWaves = S <= 32 ? 2 : 1;
+// S = 48 on 3 x 3 tiles of 16x16x32, smaller sizes and S = 64 on 32x32x16
+template <int S, class P>
+__device__ __forceinline__ void grid_mfma_any(const KernelArgs& a, int s, int k0, int k1) {
+    if constexpr (S == 48)
+        grid_mfma16_body<S, P>(a, s, k0, k1);
+    else
+        grid_mfma_body<S, P>(a, s, k0, k1);
+}
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_kernel(KernelArgs a) {
-    grid_mfma_body<S, P>(a, a.order[blockIdx.x], 0, -1);
+    grid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1);
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_queue_kernel(KernelArgs a) {
-    run_queue(a, [&](int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
+    run_queue(a, [&](int s, int k0, int k1) { grid_mfma_any<S, P>(a, s, k0, k1); });
 }
 
-// S <= 32: 16x16x32 tiles, two waves per SIMD; S = 48, 64: the wide form
+// S <= 32: 16x16x32 tiles, two waves per SIMD; S = 48: 3 row tiles and two
+// k steps of 16x16x32, one wave per SIMD; S = 64: the wide form
 template <int S>
 constexpr int kDegridWaves = S <= 32 ? 2 : 1;
 template <int S, class P>
 __device__ __forceinline__ void degrid_mfma_any(const KernelArgs& a, int s, int k0, int k1) {
-    if constexpr (S <= 32)
+    if constexpr (S <= 48)
         degrid_mfma_body<S, P>(a, s, k0, k1);
     else
         degrid_mfma_wide_body<S, P>(a, s, k0, k1);
@@ -772,7 +983,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
 
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_batch_kernel(BatchArgs b) {
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_body<S, P>(a, s, k0, k1); });
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_any<S, P>(a, s, k0, k1); });
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_batch_kernel(BatchArgs b) {

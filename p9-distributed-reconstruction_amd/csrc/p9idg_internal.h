@@ -84,8 +84,11 @@ struct KernelArgs {
     int32_t mode;
     // nonzero: channels are uniformly spaced in wavenumber and C % 8 == 0, so
     // each aligned run of 8 visibilities is one timestep's 8 consecutive
-    // channels and the MFMA kernels step their phasors channel to channel
+    // channels and the MFMA gridder steps its phasors channel to channel
     int32_t chan_rec;
+    // nonzero: channels are uniformly spaced in wavenumber (any C): the
+    // degridder, one timestep per lane, steps its phasors channel to channel
+    int32_t chan_step;
     // -1.0 as BF16 in the low half, as BF16 in the high half, and as FP16 in
     // the low half (0x0000BF80, 0xBF800000, 0x0000BC00): the MFMA kernels'
     // hi/lo split takes its residual a - hi in one dot2 (BF16) or mixed FMA

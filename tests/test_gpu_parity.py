@@ -99,6 +99,7 @@ CASES = [
     (24, 8, 4, True, 10, 40, 6),
     (32, 8, 4, True, 12, 50, 8),
     (48, 8, 2, True, 8, 30, 4),
+    (48, 8, 4, True, 10, 40, 8),  # 3 x 3 tiles of 16 with channel stepping
     (64, 8, 2, True, 8, 30, 4),
 ]
 
@@ -126,20 +127,23 @@ def test_subgrid_sizes_and_wstack(O, S, K, L, wst, n_ant, T, C, kernel):
 
 
 REC_CASES = [
-    # channels, frequency jitter (Hz), work-item size: C % 8 == 0 with even
-    # spacing steps the phasors channel to channel, re-anchored every 8
-    # channels and at every item start (C = 24 items start mid-timestep);
-    # uneven spacing takes the direct evaluation
-    (16, 0.0, 160),
-    (24, 0.0, 96),
-    (8, 3.0e5, 64),
+    # subgrid, channels, frequency jitter (Hz), work-item size: C % 8 == 0
+    # with even spacing steps the phasors channel to channel, re-anchored
+    # every 8 channels and at every item start (C = 24 items start
+    # mid-timestep); uneven spacing takes the direct evaluation
+    (32, 16, 0.0, 160),
+    (32, 24, 0.0, 96),
+    (32, 8, 3.0e5, 64),
+    (48, 16, 0.0, 160),
+    (48, 24, 0.0, 96),
+    (48, 8, 3.0e5, 64),
 ]
 
 
-@pytest.mark.parametrize("C,jitter,chunk", REC_CASES)
+@pytest.mark.parametrize("S,C,jitter,chunk", REC_CASES)
 @pytest.mark.parametrize("kernel", ["mfma_bf16", "mfma_fp16"])
-def test_channel_stepping_cases(O, C, jitter, chunk, kernel):
-    ob = obs.custom(G=256, S=32, K=8, L=4, n_ant=10, T=40, C=C, f0=1.0e9, f1=1.05e9, seed=70 + C,
+def test_channel_stepping_cases(O, S, C, jitter, chunk, kernel):
+    ob = obs.custom(G=256, S=S, K=8, L=4, n_ant=10, T=40, C=C, f0=1.0e9, f1=1.05e9, seed=70 + C + S,
                     core_diam=1200.0, noise=0.01)
     if jitter:
         ob.freq = ob.freq + jitter * np.sin(np.arange(C) * 1.7)
@@ -155,7 +159,7 @@ def test_channel_stepping_cases(O, C, jitter, chunk, kernel):
         plan.set_launch(True, chunk, chunk, 0)
         e_g = rel_err(plan.gridw(ob.vis), ref_l)
         e_d = rel_err(plan.degridw(model), ref_v)
-    print(kernel, C, jitter, chunk, "gridw %.2e degridw %.2e" % (e_g, e_d))
+    print(kernel, S, C, jitter, chunk, "gridw %.2e degridw %.2e" % (e_g, e_d))
     assert e_g <= TOL and e_d <= TOL
 
 
