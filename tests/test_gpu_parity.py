@@ -127,16 +127,24 @@ def test_subgrid_sizes_and_wstack(O, S, K, L, wst, n_ant, T, C, kernel):
 
 
 REC_CASES = [
-    # subgrid, channels, frequency jitter (Hz), work-item size: C % 8 == 0
-    # with even spacing steps the phasors channel to channel, re-anchored
-    # every 8 channels and at every item start (C = 24 items start
-    # mid-timestep); uneven spacing takes the direct evaluation
+    # subgrid, channels, frequency jitter (Hz), work-item size: with even
+    # spacing and C % 8 == 0 the gridder steps its phasors channel to
+    # channel, re-anchored every 8 visibilities and at every item start
+    # (items of 96 start mid-timestep for C = 24, 11, 14); the degridder
+    # steps them for any C with even spacing; uneven spacing (and, in the
+    # gridder, C % 8 != 0) takes the direct evaluation
     (32, 16, 0.0, 160),
     (32, 24, 0.0, 96),
     (32, 8, 3.0e5, 64),
+    (32, 9, 0.0, 160),
+    (32, 11, 0.0, 96),
+    (32, 13, 0.0, 64),
+    (32, 5, 0.0, 64),
     (48, 16, 0.0, 160),
     (48, 24, 0.0, 96),
     (48, 8, 3.0e5, 64),
+    (48, 14, 0.0, 96),
+    (48, 9, 0.0, 160),
 ]
 
 
