@@ -158,11 +158,12 @@ constexpr double kInvTwoPi = 1.0 / kTwoPi;
 constexpr float kTwoPiF = 6.2831853071795864769f;
 
 // ---------------------------------------------------------------------------
-// Gridder.  One 256-thread workgroup per subgrid.  For S <= 32 the subgrid is
-// one 32 x 32 MFMA tile and the 4 waves split the subgrid's visibilities (16
-// per MFMA step, 8 per lane group), each keeping a full set of accumulators;
-// for S = 48, 64 the subgrid is 2 x 2 tiles and wave w owns tile w over all
-// visibilities.  Partial subgrids are summed in LDS.
+// Gridder on 32x32x16 MFMA tiles, used for S = 8, 24 and 64.  One 256-thread
+// workgroup per subgrid.  For S <= 32 the subgrid is one 32 x 32 MFMA tile
+// and the 4 waves split the subgrid's visibilities (16 per MFMA step, 8 per
+// lane group), each keeping a full set of accumulators; for S = 64 the
+// subgrid is 2 x 2 tiles and wave w owns tile w over all visibilities.
+// Partial subgrids are summed in LDS.
 template <int S, class P>
 __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
     constexpr int NT = (S + 31) / 32;  // tiles per axis
@@ -346,13 +347,13 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 }
 
 // ---------------------------------------------------------------------------
-// Gridder for S = 48 on 16x16x32 MFMA tiles: the subgrid is 3 x 3 tiles of
-// 16 x 16 (no padding to 64).  One 256-thread workgroup per work item; a step
-// is 32 visibilities, the 4 waves take alternate steps and each accumulates
-// all 9 tiles.  Lane l = 16 g + c: c is the row (A operand) and column (B
+// Gridder for S = 16, 32, 48 on 16x16x32 MFMA tiles: the subgrid is NT x NT
+// tiles of 16 x 16 (S = 48 is not padded to 64).  One 256-thread workgroup
+// per work item; a step is 32 visibilities, the 4 waves take alternate steps
+// and each accumulates all NT x NT tiles.  Lane l = 16 g + c: c is the row (A operand) and column (B
 // operand) within a tile, g picks the lane's 8 visibilities of the step.
-// Per step the x-phasors of the 3 column tiles are built once, then each row
-// tile's V * y-phasor operands are built and multiplied into its 3 tiles.
+// Per step the x-phasors of the NT column tiles are built once, then each row
+// tile's V * y-phasor operands are built and multiplied into its NT tiles.
 template <int S, class P>
 __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, int k0, int k1) {
     constexpr int NT = S / 16;  // tiles per axis
@@ -944,10 +945,11 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
 // two waves per SIMD up to S = 48; the S = 64 gridder needs more registers
 template <int S>
 constexpr int kGridWaves = S <= 32 ? 2 : 1;
-// S = 48 on 3 x 3 tiles of 16x16x32, smaller sizes and S = 64 on 32x32x16
+// multiples of 16 up to 48 on 16x16x32 tiles (as many as the subgrid holds),
+// S = 8, 24 and 64 on 32x32x16
 template <int S, class P>
 __device__ __forceinline__ void grid_mfma_any(const KernelArgs& a, int s, int k0, int k1) {
-    if constexpr (S == 48)
+    if constexpr (S % 16 == 0 && S <= 48)
         grid_mfma16_body<S, P>(a, s, k0, k1);
     else
         grid_mfma_body<S, P>(a, s, k0, k1);

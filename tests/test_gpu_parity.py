@@ -133,6 +133,7 @@ REC_CASES = [
     # (items of 96 start mid-timestep for C = 24, 11, 14); the degridder
     # steps them for any C with even spacing; uneven spacing (and, in the
     # gridder, C % 8 != 0) takes the direct evaluation
+    (16, 24, 0.0, 96),
     (32, 16, 0.0, 160),
     (32, 24, 0.0, 96),
     (32, 8, 3.0e5, 64),
