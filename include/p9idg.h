@@ -262,7 +262,11 @@ int p9idg_gridded_visibilities(p9idg_plan* plan, int64_t* n);
 /* The plans keep their observations and buffers; a batch groups them by
  * (subgrid size, kernel) and runs each group's work items -- the K-split
  * subgrids of all its observations, longest first -- as one persistent
- * work-queued launch per direction, on the batch's own stream. */
+ * work-queued launch per direction, on the batch's own stream.  A batch run
+ * (exec, capture's eager run, launch) starts after the work already queued on
+ * every member plan and the plans' later work waits for it.  The plans must
+ * outlive the batch; after p9idg_set_observation on a member plan the batch
+ * returns P9IDG_ERR_STATE until it is prepared (and captured) again. */
 typedef struct p9idg_batch p9idg_batch;
 int p9idg_batch_create(p9idg_plan* const* plans, int32_t nr_plans, p9idg_batch** batch);
 int p9idg_batch_destroy(p9idg_batch* batch);

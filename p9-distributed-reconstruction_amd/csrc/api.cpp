@@ -26,7 +26,10 @@ int fail(int status, const std::string& msg) {
     set_error(msg);
     return status;
 }
+// A reported HIP failure is also cleared from the runtime's last-error
+// state, so a later launch check does not report it again.
 int hip_fail(hipError_t e, const char* what) {
+    (void)hipGetLastError();
     return fail(P9IDG_ERR_HIP, std::string("HIP error ") + hipGetErrorName(e) + " (" +
                                    hipGetErrorString(e) + ") in " + what);
 }
@@ -48,6 +51,11 @@ struct p9idg_plan {
     bool have_obs = false;
     int32_t B = 0, T = 0, C = 0;
     int64_t nsub = 0;
+    // element counts the observation-shaped buffers are allocated for (0: not
+    // allocated); kept apart from B, T, C and nsub, which describe the last
+    // observation set successfully
+    size_t alloc_uvw = 0, alloc_wn = 0, alloc_sub = 0, alloc_vis = 0;
+    uint64_t obs_gen = 0;  // bumped by every p9idg_set_observation past argument checks
     bool plain_adder_ok = true;  // every subgrid passes Adder.cs:30-31
     bool chan_uniform = false;   // wavenumbers equally spaced (to 1e-12 relative)
     int64_t gridded_vis = 0;
@@ -131,10 +139,12 @@ int dev_alloc(T** p, size_t count) {
     if (*p) return P9IDG_OK;
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, padded(count * sizeof(T)));
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
         return fail(P9IDG_ERR_ALLOC, std::string("device allocation of ") +
                                          std::to_string(count * sizeof(T)) +
                                          " bytes failed: " + hipGetErrorString(e));
+    }
     *p = static_cast<T*>(q);
     return P9IDG_OK;
 }
@@ -255,8 +265,10 @@ int ensure_staging(p9idg_plan* p, size_t bytes) {
     p->d_staging = nullptr;
     p->staging_bytes = 0;
     hipError_t e = hipMalloc(&p->d_staging, padded(bytes));
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
         return fail(P9IDG_ERR_ALLOC, std::string("staging allocation failed: ") + hipGetErrorString(e));
+    }
     p->staging_bytes = bytes;
     return P9IDG_OK;
 }
@@ -798,29 +810,43 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
         gridded += (int64_t)m.time_sample_count * C;
     }
     DeviceGuard dg(p->device);
-    // drop the previous observation's buffers if the shape changed
+    p->obs_gen++;
+    // drop every buffer whose allocated size differs from the new shape (by
+    // the recorded sizes, so a call that failed part-way leaves nothing
+    // undersized behind)
     drop_graph(p);
-    if (p->have_obs && ((size_t)p->B * p->T * p->C != (size_t)B * T * C)) {
+    const size_t n_uvw = (size_t)B * T * 3, n_sub = (size_t)std::max<int64_t>(nsub, 1), n_vis = (size_t)B * T * C;
+    if (p->alloc_vis != n_vis) {
         dev_free(p->d_vis);
         dev_free(p->d_model_vis);
         dev_free(p->d_flags);
         dev_free(p->d_resid);
         p->have_flags = false;
+        p->alloc_vis = 0;
     }
-    if (p->have_obs && (size_t)p->B * p->T != (size_t)B * T) dev_free(p->d_uvw);
-    if (p->have_obs && p->C != C) dev_free(p->d_wavenumbers);
-    if (p->have_obs && p->nsub != nsub) {
+    if (p->alloc_uvw != n_uvw) {
+        dev_free(p->d_uvw);
+        p->alloc_uvw = 0;
+    }
+    if (p->alloc_wn != (size_t)C) {
+        dev_free(p->d_wavenumbers);
+        p->alloc_wn = 0;
+    }
+    if (p->alloc_sub != n_sub) {
         dev_free(p->d_meta);
         dev_free(p->d_order);
+        p->alloc_sub = 0;
     }
     p->have_obs = false;
     int st;
-    if ((st = dev_alloc(&p->d_uvw, (size_t)B * T * 3))) return st;
+    if ((st = dev_alloc(&p->d_uvw, n_uvw))) return st;
+    p->alloc_uvw = n_uvw;
     if ((st = dev_alloc(&p->d_wavenumbers, (size_t)C))) return st;
-    if ((st = dev_alloc(&p->d_meta, (size_t)std::max<int64_t>(nsub, 1)))) return st;
-    if ((st = dev_alloc(&p->d_order, (size_t)std::max<int64_t>(nsub, 1)))) return st;
-    if ((st = dev_alloc(&p->d_vis, (size_t)B * T * C))) return st;
-    if ((st = dev_alloc(&p->d_model_vis, (size_t)B * T * C))) return st;
+    p->alloc_wn = (size_t)C;
+    if ((st = dev_alloc(&p->d_meta, n_sub)) || (st = dev_alloc(&p->d_order, n_sub))) return st;
+    p->alloc_sub = n_sub;
+    if ((st = dev_alloc(&p->d_vis, n_vis)) || (st = dev_alloc(&p->d_model_vis, n_vis))) return st;
+    p->alloc_vis = n_vis;
     // MathFunctions.cs:69-78: k = 2 pi f / c, (2.0f promoted to double)
     std::vector<double> k(C);
     for (int i = 0; i < C; i++) k[i] = ((2.0 * M_PI) * freq[i]) / 299792458.0;
@@ -1338,6 +1364,9 @@ int p9idg_timing_totals(p9idg_plan* p, double* grid_ms, int64_t* grid_launches,
 // ---- batches: several observations per persistent launch --------------------
 struct p9idg_batch {
     std::vector<p9idg_plan*> plans;
+    std::vector<uint64_t> obs_gen;     // each plan's observation generation at prepare
+    std::vector<hipEvent_t> plan_ev;   // per plan: its stream's work before a batch run
+    hipEvent_t done = nullptr;         // end of the last batch run, awaited by the plans
     int device = 0;
     hipStream_t stream = nullptr;
     struct Group {
@@ -1377,9 +1406,32 @@ void batch_free_groups(p9idg_batch* b) {
     b->prepared = false;
 }
 
+// The batch works on the plans' buffers on its own stream: it starts after
+// the work already queued on every plan's stream, and every plan's later
+// work starts after it.
+int batch_check(const p9idg_batch* b) {
+    if (!b->prepared) return fail(P9IDG_ERR_STATE, "batch not prepared (p9idg_batch_prepare)");
+    for (size_t i = 0; i < b->plans.size(); i++)
+        if (b->plans[i]->obs_gen != b->obs_gen[i])
+            return fail(P9IDG_ERR_STATE, "a plan's observation changed after p9idg_batch_prepare");
+    return P9IDG_OK;
+}
+int batch_order_in(p9idg_batch* b) {
+    for (size_t i = 0; i < b->plans.size(); i++) {
+        P9_HIP(hipEventRecord(b->plan_ev[i], b->plans[i]->stream));
+        P9_HIP(hipStreamWaitEvent(b->stream, b->plan_ev[i], 0));
+    }
+    return P9IDG_OK;
+}
+int batch_order_out(p9idg_batch* b) {
+    P9_HIP(hipEventRecord(b->done, b->stream));
+    for (p9idg_plan* p : b->plans) P9_HIP(hipStreamWaitEvent(p->stream, b->done, 0));
+    return P9IDG_OK;
+}
+
 // Zero fills and one persistent launch per (subgrid size, kernel) group.
 int batch_run(p9idg_batch* b, int what, bool capturing) {
-    if (!b->prepared) return fail(P9IDG_ERR_STATE, "batch not prepared (p9idg_batch_prepare)");
+    if (int st = batch_check(b)) return st;
     for (p9idg_plan* p : b->plans) mark_unknown(p);
     if (!capturing) P9_HIP(hipEventRecord(b->ev[0], b->stream));
     if (what & 1) {
@@ -1434,6 +1486,14 @@ int p9idg_batch_create(p9idg_plan* const* plans, int32_t n, p9idg_batch** out) {
         return hip_fail(e, "hipStreamCreate");
     }
     for (auto& ev : b->ev) (void)hipEventCreate(&ev);
+    b->plan_ev.assign(n, nullptr);
+    b->obs_gen.assign(n, 0);
+    bool ok = hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    for (auto& ev : b->plan_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        p9idg_batch_destroy(b);
+        return fail(P9IDG_ERR_HIP, "hipEventCreate failed");
+    }
     *out = b;
     return P9IDG_OK;
 }
@@ -1445,6 +1505,9 @@ int p9idg_batch_destroy(p9idg_batch* b) {
     batch_free_groups(b);
     for (auto& ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : b->plan_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->done) (void)hipEventDestroy(b->done);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return P9IDG_OK;
@@ -1516,6 +1579,7 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
         g.n_grid = (int32_t)ig.size();
         g.n_degrid = (int32_t)id.size();
     }
+    for (size_t i = 0; i < b->plans.size(); i++) b->obs_gen[i] = b->plans[i]->obs_gen;
     b->prepared = true;
     return P9IDG_OK;
 }
@@ -1524,7 +1588,10 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
 int p9idg_batch_exec(p9idg_batch* b, int32_t what) {
     if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
     DeviceGuard dg(b->device);
-    return batch_run(b, what, false);
+    if (int st = batch_check(b)) return st;
+    if (int st = batch_order_in(b)) return st;
+    if (int st = batch_run(b, what, false)) return st;
+    return batch_order_out(b);
 }
 
 int p9idg_batch_capture(p9idg_batch* b, int32_t what) {
@@ -1534,6 +1601,8 @@ int p9idg_batch_capture(p9idg_batch* b, int32_t what) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->graph_exec = nullptr;
     b->graph = nullptr;
+    if (int st = batch_check(b)) return st;
+    if (int st = batch_order_in(b)) return st;
     if (int st = batch_run(b, what, false)) return st;  // warm: occupancy figures, first launches
     P9_HIP(hipStreamSynchronize(b->stream));
     P9_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1555,8 +1624,10 @@ int p9idg_batch_launch(p9idg_batch* b) {
     if (!b) return fail(P9IDG_ERR_ARGUMENT, "batch is NULL");
     if (!b->graph_exec) return fail(P9IDG_ERR_STATE, "no captured batch graph");
     DeviceGuard dg(b->device);
+    if (int st = batch_check(b)) return st;
+    if (int st = batch_order_in(b)) return st;
     P9_HIP(hipGraphLaunch(b->graph_exec, b->stream));
-    return P9IDG_OK;
+    return batch_order_out(b);
 }
 
 int p9idg_batch_sync(p9idg_batch* b) {

@@ -35,8 +35,11 @@ struct Guard {
 template <typename T>
 int alloc(T** p, size_t n) {
     *p = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+    if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();  // reported here, not by a later launch check
         return fail(P9IDG_ERR_ALLOC, "device allocation failed");
+    }
     return P9IDG_OK;
 }
 template <typename T>

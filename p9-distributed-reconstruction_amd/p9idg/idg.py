@@ -381,6 +381,10 @@ class Batch:
         arr = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
         self._h = ctypes.c_void_p()
         check(_lib.idg().p9idg_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(self.plans), ctypes.byref(self._h)))
+        self.prepare(grid_mode, degrid_mode)
+
+    def prepare(self, grid_mode=_lib.GRID_WSTACK, degrid_mode=_lib.DEGRID_WSTACK):
+        """(Re)builds the merged work queues, e.g. after a plan's observation changed."""
         check(_lib.idg().p9idg_batch_prepare(self._h, int(grid_mode), int(degrid_mode)))
 
     def exec(self, what=3):

@@ -186,6 +186,25 @@ def test_invalid_grid_error_and_wstack_skip(O):
         assert rel_err(plan.gridw(ob.vis), O.gridw(oc, meta, ob.vis, ob.uvw, ob.freq)) <= TOL
 
 
+def test_reobserve_after_failed_set_observation(O):
+    """A set_observation that fails part-way (here the visibility buffers do
+    not fit in device memory) leaves no undersized buffer behind: the next,
+    larger observation grids correctly."""
+    ob1 = obs.custom(G=128, S=16, K=4, L=1, n_ant=5, T=8, C=2, seed=11)
+    ob2 = obs.custom(G=128, S=16, K=4, L=1, n_ant=8, T=16, C=5, seed=12)
+    part2 = ob2.partition()
+    with p9idg.Plan(ob2.c) as plan:
+        plan.set_observation(ob1.uvw, ob1.freq, ob1.partition())
+        B, T, C = 1000, 1000, 50000  # 5e10 visibilities, 400 GB per FP32 buffer
+        with pytest.raises(p9idg.P9IDGError) as e:
+            plan.set_observation(np.zeros((B, T, 3)), np.linspace(1.0e9, 1.1e9, C), part2.subgrids[:0])
+        assert e.value.status == p9idg._lib.ERR_ALLOC
+        plan.set_observation(ob2.uvw, ob2.freq, part2)
+        got = plan.gridw(ob2.vis)
+    ref = O.gridw(oracle_constants(ob2.c), part2.subgrids, ob2.vis, ob2.uvw, ob2.freq)
+    assert rel_err(got, ref) <= TOL
+
+
 def test_bad_metadata_rejected():
     ob = obs.custom(G=128, S=16, K=4, L=2, n_ant=6, T=10, C=3, seed=6)
     part = ob.partition()
@@ -372,15 +391,39 @@ def test_batched_uneven_instances_match_oracle(O):
                 assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
                 assert rel_err(p.download_model_vis(), ref_v) <= TOL
             batch.capture(3)
-            for rep in range(2):
+            for rep in range(3):
                 for p in plans:  # eager work between replays
                     p.exec_grid(p9idg.GRID_PLAIN)
                     p.sync()
+                    if rep == 2:  # left queued: the batch must start after it
+                        p.exec_grid(p9idg.GRID_WSTACK)
                 batch.launch()
-                batch.sync()
+                if rep < 2:
+                    batch.sync()
+                # without batch.sync() the plans' downloads must wait for the batch
                 for p, (ref_l, ref_v) in zip(plans, refs):
                     assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL, rep
                     assert rel_err(p.download_model_vis(), ref_v) <= TOL, rep
+
+            # a member plan's new observation invalidates the batch until prepared again
+            ob = obs.custom(G=256, S=24, K=8, L=4, n_ant=8, T=24, C=7, seed=65, wstep=obsv[0].c.WStep)
+            part = ob.partition()
+            oc = oracle_constants(ob.c)
+            ref_l = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
+            ref_v = O.degridw(oc, part.subgrids, ref_l.sum(axis=0), ob.uvw, ob.freq)
+            plans[0].set_observation(ob.uvw, ob.freq, part)
+            plans[0].upload_vis(ob.vis)
+            plans[0].upload_model_grid(ref_l.sum(axis=0).astype(np.complex64))
+            for call in (lambda: batch.exec(3), batch.launch):
+                with pytest.raises(p9idg.P9IDGError) as e:
+                    call()
+                assert e.value.status == p9idg._lib.ERR_STATE
+            batch.prepare()
+            batch.exec(3)
+            assert rel_err(plans[0].download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
+            assert rel_err(plans[0].download_model_vis(), ref_v) <= TOL
+            for p, (ref_l, ref_v) in zip(plans[1:], refs[1:]):
+                assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
     finally:
         for p in plans:
             p.close()
