@@ -200,6 +200,7 @@ int p9psf_max_sidelobe(const float* psf, int32_t n, int32_t factor, float* out) 
 
 int p9psf_max_lipschitz(const float* psf, int32_t rows, int32_t cols, double* out) {
     if (!psf || !out) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
+    if (int st = check_dims(rows, cols)) return st;
     double s = 0.0;
     for (size_t i = 0; i < (size_t)rows * cols; i++) {
         const float sq = psf[i] * psf[i];
@@ -314,8 +315,10 @@ extern "C" int p9psf_amap(int32_t device, const float* psf, int32_t rows, int32_
     float* d = nullptr;
     if (int st = alloc(&d, (size_t)height * width)) return st;
     int st = amap_device(psf, rows, cols, height, width, d, nullptr);
-    if (!st && hipMemcpy(out, d, sizeof(float) * height * width, hipMemcpyDeviceToHost) != hipSuccess)
-        st = fail(P9IDG_ERR_HIP, "Lipschitz map download");
+    if (!st) {
+        const hipError_t e = hipMemcpy(out, d, sizeof(float) * height * width, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "Lipschitz map download");
+    }
     release(d);
     return st;
 }
@@ -424,8 +427,8 @@ int p9cd_create(int32_t device, int32_t height, int32_t width, const float* psf,
     float m = 0.f;
     for (float v : psf2) m = std::fmax(m, v);
     c->max_lipschitz = m;
-    if (hipMemcpy(c->d_psf2, psf2.data(), sizeof(float) * psf2.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return bail(fail(P9IDG_ERR_HIP, "psf2 upload"));
+    if (hipError_t e = hipMemcpy(c->d_psf2, psf2.data(), sizeof(float) * psf2.size(), hipMemcpyHostToDevice))
+        return bail(hip_fail(e, "psf2 upload"));
     if ((st = amap_device(psf, rows, cols, height, width, c->d_amap, c->stream))) return bail(st);
     *out = c;
     return P9IDG_OK;

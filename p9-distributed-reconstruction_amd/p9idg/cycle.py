@@ -60,7 +60,8 @@ class PSF:
     def Cut(psf, factor=2):
         psf = _f32(psf)
         n = psf.shape[0]
-        out = np.zeros((n // factor, n // factor), dtype=np.float32)
+        m = n // factor if factor > 0 else 0  # the library rejects factor <= 0
+        out = np.zeros((m, m), dtype=np.float32)
         check(_lib.idg().p9psf_cut(ptr(psf), n, int(factor), ptr(out)))
         return out
 

@@ -37,6 +37,23 @@ def test_cut_and_sidelobe(O):
     assert O.psf_max_lipschitz(psf) == pytest.approx(float(np.sum((psf * psf).astype(np.float64))), rel=1e-15)
 
 
+@pytest.mark.parametrize("n,factor", [(16, 4), (17, 2), (33, 3), (12, 1), (9, 9)])
+def test_native_psf_host_helpers_match_oracle(O, n, factor):
+    """PSF.Cut, CalcMaxSidelobe and CalcMaxLipschitz of libp9idg (host code,
+    no GPU) against the oracle, bit for bit, odd and even sizes."""
+    rng = np.random.default_rng(n * 10 + factor)
+    psf = rng.standard_normal((n, n)).astype(np.float32)
+    assert np.array_equal(p9idg.PSF.Cut(psf, factor), O.psf_cut(psf, factor))
+    assert p9idg.PSF.CalcMaxSidelobe(psf, factor) == O.psf_max_sidelobe(psf, factor)
+    assert p9idg.PSF.CalcMaxLipschitz(psf) == O.psf_max_lipschitz(psf)
+    with pytest.raises(p9idg.P9IDGError) as e:
+        p9idg.PSF.Cut(psf, 0)
+    assert e.value.status == p9idg._lib.ERR_ARGUMENT
+    with pytest.raises(p9idg.P9IDGError) as e:
+        p9idg.PSF.CalcMaxLipschitz(np.zeros((0, 4), dtype=np.float32))
+    assert e.value.status == p9idg._lib.ERR_ARGUMENT
+
+
 @pytest.mark.parametrize("ph,pw,H,W", [(6, 6, 20, 20), (5, 5, 17, 17), (8, 4, 19, 23)])
 def test_amap_is_psf_energy_inside_the_image(O, ph, pw, H, W):
     rng = np.random.default_rng(ph * 7 + W)
