@@ -203,6 +203,7 @@ def run_cycle(args):
     c = ob.c
     G = c.GridSize
     plan = p9idg.Plan(c, device=0, kernel=args.kernel)
+    plan.set_deterministic(args.deterministic)
     plan.set_observation(ob.uvw, ob.freq, part)
     plan.upload_vis(ob.vis)
     if not args.no_autotune:
@@ -284,6 +285,7 @@ def run_cycle(args):
         "config": {
             "workload": "large + major cycles: psf cut 16, lambda 1.0, alpha 0.01, 30000 iterations, epsilon 1e-5",
             "kernel": args.kernel,
+            "deterministic_adder": bool(args.deterministic),
             "cycles_run": len(stats),
             "deconvolution_iterations": total_iters,
             "deconvolution_ms": cd_ms,
@@ -457,6 +459,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--workload", default="large", choices=["large", "batch", "cycle", "cd"])
     ap.add_argument("--cycles", type=int, default=10, help="major cycles of --workload cycle")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="ordered subgrid sums (large and cycle workloads): run-to-run identical grids")
     ap.add_argument("--full-accuracy", action="store_true",
                     help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
@@ -490,6 +494,7 @@ def main():
     log("partition: %d subgrids in %.3f s" % (part.total, t_part))
 
     plan = p9idg.Plan(c, device=0, kernel=args.kernel)
+    plan.set_deterministic(args.deterministic)
     plan.set_observation(ob.uvw, ob.freq, part)
     plan.upload_vis(ob.vis)
     plan.upload_image(model_image(ob))
@@ -571,6 +576,7 @@ def main():
         "config": {
             "workload": "large: 3072^2 image, 32 w-layers, 2016 baselines x 2000 timesteps x 8 channels, subgrid 32, kernel 8",
             "kernel": args.kernel,
+            "deterministic_adder": bool(args.deterministic),
             "gridded_visibilities": int(n_vis),
             "total_visibilities": int(ob.vis.size),
             "subgrids": int(part.total),

@@ -142,6 +142,15 @@ int p9idg_get_constants(const p9idg_plan* plan, p9idg_constants* c, int32_t* dev
 int p9idg_destroy(p9idg_plan* plan);
 int p9idg_set_kernel(p9idg_plan* plan, int32_t kernel);
 int p9idg_get_kernel(const p9idg_plan* plan, int32_t* kernel);
+/* on != 0: the gridder sums subgrids onto the grid in a fixed order (each
+ * work item's subgrid goes to a scratch slot, then every 32 x 32 grid tile
+ * adds its covering slots in (subgrid, first visibility) order), so the
+ * result is the same bit for bit from run to run, as with the serial
+ * Adder.cs:13-55 / AdderWStack.cs:11-78; off (default): float atomics, whose
+ * order varies at the 1e-7 level.  Costs one S^2 complex64 scratch subgrid
+ * per work item and one extra pass.  Not available in batches
+ * (p9idg_batch_prepare returns P9IDG_ERR_UNSUPPORTED). */
+int p9idg_set_deterministic(p9idg_plan* plan, int32_t on);
 /* Launch shape of the gridder / degridder: queue != 0 runs persistent
  * workgroups over a work queue whose items hold at most grid_chunk /
  * degrid_chunk visibilities of one subgrid (0 = whole subgrids), with

@@ -113,6 +113,20 @@ struct p9idg_plan {
     int32_t workgroups = 0;         // gridder's persistent workgroups (0: CUs x occupancy)
     int32_t workgroups_degrid = 0;  // the degridder's (its workgroups are half the size)
 
+    // deterministic adder: each gridder work item writes its subgrid to a
+    // scratch slot, then gather_slots sums the slots per grid tile in slot order
+    bool deterministic = false;
+    float2* d_scratch = nullptr;
+    size_t alloc_scratch = 0;  // float2 elements
+    bool gather_valid = false;
+    int32_t gather_mode = -1;
+    GatherBin* d_bins = nullptr;
+    GatherChunk* d_chunks = nullptr;
+    int32_t* d_bin_slots = nullptr;
+    SlotCorner* d_corners = nullptr;
+    float2* d_partial = nullptr;  // one 32 x 32 tile per chunk
+    int32_t n_bins = 0, n_chunks = 0, n_slots = 0;
+
     hipfftHandle fft_one = 0;
     bool have_fft_one = false;
 
@@ -326,15 +340,17 @@ int span_end(p9idg_plan* p) {
 }
 
 // Splits every subgrid's visibilities into items of at most `chunk` (rounded
-// to multiples of 32) and orders them longest first.
+// to multiples of 32; one item per subgrid for chunk 0) and orders them
+// longest first.  Slots number the items in (subgrid, k0) order, so with
+// chunk 0 an item's slot is its subgrid.
 std::vector<WorkItem> make_items(const std::vector<p9idg_subgrid>& meta,
                                  const std::vector<int32_t>& order, int C, int chunk) {
-    std::vector<WorkItem> items;
-    items.reserve(meta.size());
-    for (int32_t s : order) {
+    std::vector<std::vector<WorkItem>> per(meta.size());
+    int32_t slot = 0;
+    for (int32_t s = 0; s < (int32_t)meta.size(); s++) {
         const int K = meta[s].time_sample_count * C;
         if (K <= 0) {
-            items.push_back({s, 0, 0, 0});
+            per[s].push_back({s, 0, 0, 0, slot++});
             continue;
         }
         const int n = chunk > 0 ? (K + chunk - 1) / chunk : 1;
@@ -342,17 +358,25 @@ std::vector<WorkItem> make_items(const std::vector<p9idg_subgrid>& meta,
         for (int i = 1; i <= n; i++) {
             int k1 = i == n ? K : (int)(((int64_t)K * i / n + 31) / 32 * 32);
             if (k1 > K) k1 = K;
-            if (k1 > k0) items.push_back({s, k0, k1, 0});
+            if (k1 > k0) per[s].push_back({s, k0, k1, 0, slot++});
             k0 = k1;
         }
     }
+    std::vector<WorkItem> items;
+    items.reserve(slot);
+    for (int32_t s : order)
+        for (const WorkItem& w : per[s]) items.push_back(w);
     std::stable_sort(items.begin(), items.end(),
                      [](const WorkItem& a, const WorkItem& b) { return a.k1 - a.k0 > b.k1 - b.k0; });
     return items;
 }
 
+void drop_graph(p9idg_plan* p);
+
 int upload_items(p9idg_plan* p) {
     DeviceGuard dg(p->device);
+    drop_graph(p);  // a captured step holds the old item arrays
+    p->gather_valid = false;
     dev_free(p->d_items_grid);
     dev_free(p->d_items_degrid);
     p->nitems_grid = p->nitems_degrid = 0;
@@ -410,6 +434,99 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
     return a;
 }
 
+// Scratch and tile bins of the deterministic adder for `mode`: every gridder
+// work item (the queue's items, or one per subgrid without the queue) owns
+// one S x S scratch slot; each 32 x 32 tile of the target lists, in slot
+// order, the slots whose box (the adder's corner after the negative-w
+// mirror; the adder's skips excluded) overlaps it.
+int prepare_gather(p9idg_plan* p, int mode) {
+    const int S = p->c.subgrid_size, G = p->c.grid_size, L = p->c.wlayer_count;
+    const std::vector<WorkItem> items =
+        make_items(p->meta_host, p->order_host, p->C, p->use_queue ? p->grid_chunk : 0);
+    const size_t need = items.size() * (size_t)S * S;
+    if (p->alloc_scratch != need) {
+        dev_free(p->d_scratch);
+        p->alloc_scratch = 0;
+        if (int st = dev_alloc(&p->d_scratch, std::max<size_t>(need, 1))) return st;
+        p->alloc_scratch = need;
+    }
+    if (p->gather_valid && p->gather_mode == mode) return P9IDG_OK;
+    const int nt = (G + kGatherTile - 1) / kGatherTile, nl = mode == P9IDG_GRID_WSTACK ? L : 1;
+    std::vector<SlotCorner> corner(items.size());
+    std::vector<int32_t> layer(items.size(), -1);  // -1: the adder skips the slot
+    for (const WorkItem& w : items) {
+        const p9idg_subgrid& m = p->meta_host[w.sidx];
+        int X = m.u_pixel, Y = m.v_pixel, W = 0;
+        bool ok;
+        if (mode == P9IDG_GRID_PLAIN) {
+            ok = X >= 0 && X < G - S && Y >= 0 && Y < G - S;
+        } else {
+            W = m.w_lambda;
+            if (W < 0) {
+                X = G - X - S + 1;
+                Y = G - Y - S + 1;
+                W = -W - 1;
+            }
+            ok = X >= 1 && X < G - S && Y >= 1 && Y < G - S && W >= 0 && W < L;
+            if (mode == P9IDG_GRID_WSUM) W = 0;
+        }
+        corner[w.slot] = {Y, X};
+        if (ok) layer[w.slot] = W;
+    }
+    // counting sort of (tile, slot) pairs by tile; slots ascend within a tile
+    const size_t nkeys = (size_t)nl * nt * nt;
+    std::vector<int32_t> start(nkeys + 1, 0);
+    auto for_tiles = [&](int32_t s, auto&& f) {
+        const SlotCorner c = corner[s];
+        for (int ty = c.y0 / kGatherTile; ty <= (c.y0 + S - 1) / kGatherTile; ty++)
+            for (int tx = c.x0 / kGatherTile; tx <= (c.x0 + S - 1) / kGatherTile; tx++)
+                f(((size_t)layer[s] * nt + ty) * nt + tx);
+    };
+    for (int32_t s = 0; s < (int32_t)items.size(); s++)
+        if (layer[s] >= 0) for_tiles(s, [&](size_t k) { start[k + 1]++; });
+    for (size_t k = 0; k < nkeys; k++) start[k + 1] += start[k];
+    std::vector<int32_t> slots(start[nkeys]);
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int32_t s = 0; s < (int32_t)items.size(); s++)
+        if (layer[s] >= 0) for_tiles(s, [&](size_t k) { slots[fill[k]++] = s; });
+    // each tile's slot list cut into chunks of at most kGatherChunk slots
+    std::vector<GatherBin> bins;
+    std::vector<GatherChunk> chunks;
+    for (size_t k = 0; k < nkeys; k++) {
+        if (start[k + 1] == start[k]) continue;
+        const int32_t b = (int32_t)bins.size();
+        bins.push_back({(int32_t)(k / ((size_t)nt * nt)), (int32_t)(k / nt % nt), (int32_t)(k % nt),
+                        (int32_t)chunks.size(), 0});
+        for (int32_t i = start[k]; i < start[k + 1]; i += kGatherChunk)
+            chunks.push_back({b, i, std::min(i + kGatherChunk, start[k + 1])});
+        bins.back().nchunk = (int32_t)chunks.size() - bins.back().chunk0;
+    }
+    dev_free(p->d_bins);
+    dev_free(p->d_chunks);
+    dev_free(p->d_bin_slots);
+    dev_free(p->d_corners);
+    dev_free(p->d_partial);
+    if (int st = dev_alloc(&p->d_bins, std::max<size_t>(bins.size(), 1))) return st;
+    if (int st = dev_alloc(&p->d_chunks, std::max<size_t>(chunks.size(), 1))) return st;
+    if (int st = dev_alloc(&p->d_bin_slots, std::max<size_t>(slots.size(), 1))) return st;
+    if (int st = dev_alloc(&p->d_corners, std::max<size_t>(corner.size(), 1))) return st;
+    if (int st = dev_alloc(&p->d_partial, std::max<size_t>(chunks.size(), 1) * kGatherTile * kGatherTile)) return st;
+    if (!bins.empty())
+        P9_HIP(hipMemcpy(p->d_bins, bins.data(), bins.size() * sizeof(GatherBin), hipMemcpyHostToDevice));
+    if (!chunks.empty())
+        P9_HIP(hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(GatherChunk), hipMemcpyHostToDevice));
+    if (!slots.empty())
+        P9_HIP(hipMemcpy(p->d_bin_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!corner.empty())
+        P9_HIP(hipMemcpy(p->d_corners, corner.data(), corner.size() * sizeof(SlotCorner), hipMemcpyHostToDevice));
+    p->n_bins = (int32_t)bins.size();
+    p->n_chunks = (int32_t)chunks.size();
+    p->n_slots = (int32_t)items.size();
+    p->gather_mode = mode;
+    p->gather_valid = true;
+    return P9IDG_OK;
+}
+
 int do_exec_grid(p9idg_plan* p, int mode) {
     if (int st = check_obs(p)) return st;
     if (mode < P9IDG_GRID_PLAIN || mode > P9IDG_GRID_WSUM)
@@ -429,6 +546,12 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     a.grid = target;
     a.items = p->d_items_grid;
     a.nitems = p->nitems_grid;
+    if (p->deterministic) {
+        if (p->capturing && !(p->gather_valid && p->gather_mode == mode))
+            return fail(P9IDG_ERR_STATE, "deterministic adder not prepared for this mode before capture");
+        if (int st = prepare_gather(p, mode)) return st;
+        a.scratch = p->d_scratch;
+    }
     if (int st_ = record(p, 0)) return st_;
     if (int st = zero_target(p, mode, target, n)) return st;
     if (int st_ = record(p, 1)) return st_;
@@ -439,6 +562,11 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     else
         e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p, true), p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
+    if (p->deterministic) {
+        e = launch_gather_slots(p->d_bins, p->n_bins, p->d_chunks, p->n_chunks, p->d_bin_slots, p->d_corners,
+                                p->d_scratch, p->c.subgrid_size, p->c.grid_size, p->d_partial, target, p->stream);
+        if (e != hipSuccess) return hip_fail(e, "ordered subgrid sum");
+    }
     if (int st = span_end(p)) return st;
     if (int st_ = record(p, 2)) return st_;
     p->timed_grid = true;
@@ -736,6 +864,12 @@ int p9idg_destroy(p9idg_plan* p) {
     drop_graph(p);
     dev_free(p->d_items_grid);
     dev_free(p->d_items_degrid);
+    dev_free(p->d_scratch);
+    dev_free(p->d_bins);
+    dev_free(p->d_chunks);
+    dev_free(p->d_bin_slots);
+    dev_free(p->d_corners);
+    dev_free(p->d_partial);
     if (p->d_staging) (void)hipFree(p->d_staging);
     for (auto& ev : p->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -750,7 +884,15 @@ int p9idg_set_kernel(p9idg_plan* p, int32_t kernel) {
     if (kernel != P9IDG_KERNEL_FP32 && kernel != P9IDG_KERNEL_MFMA_BF16 &&
         kernel != P9IDG_KERNEL_MFMA_FP16)
         return fail(P9IDG_ERR_ARGUMENT, "unknown kernel");
+    if (p->kernel != kernel) drop_graph(p);  // a captured step holds the old kernel
     p->kernel = kernel;
+    return P9IDG_OK;
+}
+
+int p9idg_set_deterministic(p9idg_plan* p, int32_t on) {
+    if (int st = check_plan(p)) return st;
+    if ((on != 0) != p->deterministic) drop_graph(p);
+    p->deterministic = on != 0;
     return P9IDG_OK;
 }
 
@@ -1525,6 +1667,7 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
         p9idg_plan* p = b->plans[i];
         mark_unknown(p);
         if (grid_mode == P9IDG_GRID_PLAIN && !p->plain_adder_ok) return fail(P9IDG_ERR_INVALID_GRID, kMsgInvalidGrid);
+        if (p->deterministic) return fail(P9IDG_ERR_UNSUPPORTED, "the deterministic adder is not available in batches");
         if (grid_mode == P9IDG_GRID_WSTACK)
             if (int st = ensure_layers(p)) return st;
         P9_HIP(hipStreamSynchronize(p->stream));

@@ -11,11 +11,15 @@ namespace p9idg {
 namespace dev {
 
 // Adds the (normalised) subgrid tile onto the grid.  `tile` is the subgrid
-// after the inverse FFT, indexed [y][x] in FFT order.
+// after the inverse FFT, indexed [y][x] in FFT order.  With a.scratch set the
+// values that would be added at grid pixel (Y + y, X + x) are stored at
+// scratch[slot][y][x] instead (Y, X: the corner after the negative-w mirror),
+// for the ordered sum of gather_slots.
 template <int S>
 __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
-                                      const float2* tile, float norm) {
+                                      const float2* tile, float norm, int slot) {
     const int G = a.G;
+    float2* const out = a.scratch ? a.scratch + (size_t)slot * S * S : nullptr;
     if (a.mode == P9IDG_GRID_PLAIN) {
         // Adder.cs:22-47 (range validated on the host: 0 <= U,V < G-S)
         for (int p = item_tid(); p < S * S; p += kThreads) {
@@ -24,6 +28,10 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
             float2 v = cmulf(tile[ys * S + xs], a.adder_phasor[p]);
             v.x *= norm;
             v.y *= norm;
+            if (out) {
+                out[p] = v;
+                continue;
+            }
             float2* dst = a.grid + (size_t)shifted(g.V + y, G) * G + shifted(g.U + x, G);
             unsafeAtomicAdd(&dst->x, v.x);
             unsafeAtomicAdd(&dst->y, v.y);
@@ -50,6 +58,10 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
         v.x *= norm;
         v.y *= norm;
         if (neg) v.y = -v.y;
+        if (out) {
+            out[p] = v;
+            continue;
+        }
         float2* dst = layer + (size_t)shifted(Y + y, G) * G + shifted(X + x, G);
         unsafeAtomicAdd(&dst->x, v.x);
         unsafeAtomicAdd(&dst->y, v.y);

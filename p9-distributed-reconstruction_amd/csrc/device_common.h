@@ -226,7 +226,7 @@ __device__ __forceinline__ void run_queue(const KernelArgs& a, Body body) {
         const int it = s_item;
         if (it >= a.nitems) return;
         const WorkItem w = a.items[it];
-        body(w.sidx, w.k0, w.k1);
+        body(w.sidx, w.k0, w.k1, w.slot);
     }
 }
 
@@ -242,7 +242,7 @@ __device__ __forceinline__ void run_batch_queue(const BatchArgs& b, Body body) {
         if (it >= b.nitems) return;
         const WorkItem w = b.items[it];
         const KernelArgs a = b.args[w.inst];
-        body(a, w.sidx, w.k0, w.k1);
+        body(a, w.sidx, w.k0, w.k1, w.slot);
     }
 }
 

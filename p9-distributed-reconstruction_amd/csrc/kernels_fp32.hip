@@ -17,7 +17,7 @@ namespace dev {
 
 // Visibilities [k0, k1) of subgrid s (k1 < 0: all of them).
 template <int S>
-__device__ __forceinline__ void grid_fp32_body(const KernelArgs& a, int s, int k0, int k1) {
+__device__ __forceinline__ void grid_fp32_body(const KernelArgs& a, int s, int k0, int k1, int slot) {
     constexpr int NPIX = (S * S + kThreads - 1) / kThreads;
     constexpr int CHUNK = kThreads;
     __shared__ float4 s_coef[CHUNK];
@@ -78,7 +78,7 @@ __device__ __forceinline__ void grid_fp32_body(const KernelArgs& a, int s, int k
     }
     __syncthreads();
     const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, +1);
-    adder<S>(a, g, res, 1.0f / (float)(S * S));
+    adder<S>(a, g, res, 1.0f / (float)(S * S), slot);
 }
 
 template <int S>
@@ -132,11 +132,11 @@ __device__ __forceinline__ void degrid_fp32_body(const KernelArgs& a, int s, int
 
 template <int S>
 __global__ __launch_bounds__(kThreads) void grid_fp32_kernel(KernelArgs a) {
-    grid_fp32_body<S>(a, a.order[blockIdx.x], 0, -1);
+    grid_fp32_body<S>(a, a.order[blockIdx.x], 0, -1, a.order[blockIdx.x]);
 }
 template <int S>
 __global__ __launch_bounds__(kThreads) void grid_fp32_queue_kernel(KernelArgs a) {
-    run_queue(a, [&](int s, int k0, int k1) { grid_fp32_body<S>(a, s, k0, k1); });
+    run_queue(a, [&](int s, int k0, int k1, int slot) { grid_fp32_body<S>(a, s, k0, k1, slot); });
 }
 template <int S>
 __global__ __launch_bounds__(kThreads) void degrid_fp32_kernel(KernelArgs a) {
@@ -144,16 +144,16 @@ __global__ __launch_bounds__(kThreads) void degrid_fp32_kernel(KernelArgs a) {
 }
 template <int S>
 __global__ __launch_bounds__(kThreads) void degrid_fp32_queue_kernel(KernelArgs a) {
-    run_queue(a, [&](int s, int k0, int k1) { degrid_fp32_body<S>(a, s, k0, k1); });
+    run_queue(a, [&](int s, int k0, int k1, int) { degrid_fp32_body<S>(a, s, k0, k1); });
 }
 
 template <int S>
 __global__ __launch_bounds__(kThreads) void grid_fp32_batch_kernel(BatchArgs b) {
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_fp32_body<S>(a, s, k0, k1); });
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int slot) { grid_fp32_body<S>(a, s, k0, k1, slot); });
 }
 template <int S>
 __global__ __launch_bounds__(kThreads) void degrid_fp32_batch_kernel(BatchArgs b) {
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { degrid_fp32_body<S>(a, s, k0, k1); });
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int) { degrid_fp32_body<S>(a, s, k0, k1); });
 }
 
 __global__ void reset_counter_kernel(int32_t* counter) { *counter = 0; }

@@ -165,7 +165,7 @@ constexpr float kTwoPiF = 6.2831853071795864769f;
 // subgrid is 2 x 2 tiles and wave w owns tile w over all visibilities.
 // Partial subgrids are summed in LDS.
 template <int S, class P>
-__device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1) {
+__device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, int k0, int k1, int slot) {
     constexpr int NT = (S + 31) / 32;  // tiles per axis
     constexpr int WPK = 4 / (NT * NT); // waves sharing one tile's visibilities
     static_assert(NT * 32 >= S && NT <= 2, "subgrids up to 64");
@@ -343,7 +343,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
         __syncthreads();
     }
     const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, +1);
-    adder<S>(a, g, res, 1.0f / (float)(S * S));
+    adder<S>(a, g, res, 1.0f / (float)(S * S), slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -355,7 +355,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 // Per step the x-phasors of the NT column tiles are built once, then each row
 // tile's V * y-phasor operands are built and multiplied into its NT tiles.
 template <int S, class P>
-__device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, int k0, int k1) {
+__device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, int k0, int k1, int slot) {
     constexpr int NT = S / 16;  // tiles per axis
     static_assert(NT * 16 == S, "subgrid size a multiple of 16");
     constexpr int PER = 2;  // visibilities staged per thread and round
@@ -536,7 +536,7 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
         __syncthreads();
     }
     const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, +1);
-    adder<S>(a, g, res, 1.0f / (float)(S * S));
+    adder<S>(a, g, res, 1.0f / (float)(S * S), slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -948,19 +948,19 @@ constexpr int kGridWaves = S <= 32 ? 2 : 1;
 // multiples of 16 up to 48 on 16x16x32 tiles (as many as the subgrid holds),
 // S = 8, 24 and 64 on 32x32x16
 template <int S, class P>
-__device__ __forceinline__ void grid_mfma_any(const KernelArgs& a, int s, int k0, int k1) {
+__device__ __forceinline__ void grid_mfma_any(const KernelArgs& a, int s, int k0, int k1, int slot) {
     if constexpr (S % 16 == 0 && S <= 48)
-        grid_mfma16_body<S, P>(a, s, k0, k1);
+        grid_mfma16_body<S, P>(a, s, k0, k1, slot);
     else
-        grid_mfma_body<S, P>(a, s, k0, k1);
+        grid_mfma_body<S, P>(a, s, k0, k1, slot);
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_kernel(KernelArgs a) {
-    grid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1);
+    grid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1, a.order[blockIdx.x]);
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_queue_kernel(KernelArgs a) {
-    run_queue(a, [&](int s, int k0, int k1) { grid_mfma_any<S, P>(a, s, k0, k1); });
+    run_queue(a, [&](int s, int k0, int k1, int slot) { grid_mfma_any<S, P>(a, s, k0, k1, slot); });
 }
 
 // S <= 32: 16x16x32 tiles, two waves per SIMD; S = 48: 3 row tiles and two
@@ -980,16 +980,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_queue_kernel(KernelArgs a) {
-    run_queue(a, [&](int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
+    run_queue(a, [&](int s, int k0, int k1, int) { degrid_mfma_any<S, P>(a, s, k0, k1); });
 }
 
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_batch_kernel(BatchArgs b) {
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { grid_mfma_any<S, P>(a, s, k0, k1); });
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int slot) { grid_mfma_any<S, P>(a, s, k0, k1, slot); });
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_batch_kernel(BatchArgs b) {
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1) { degrid_mfma_any<S, P>(a, s, k0, k1); });
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int) { degrid_mfma_any<S, P>(a, s, k0, k1); });
 }
 
 }  // namespace dev

@@ -147,6 +147,63 @@ __global__ void zero_boxes_kernel(float2* base, const Box* boxes, int G) {
     }
 }
 
+// Deterministic subgrid sums, pass 1: one workgroup per chunk; each of the
+// tile's pixels sums the chunk's slots in slot order.  Every load reads
+// inside its slot's scratch (clamped), so the loads need no branch.
+constexpr int kGatherPer = kGatherTile * kGatherTile / 256;
+__global__ __launch_bounds__(256) void gather_chunks_kernel(const GatherBin* bins, const GatherChunk* chunks,
+                                                            const int32_t* slots, const SlotCorner* corners,
+                                                            const float2* scratch, int S, float2* partial) {
+    const GatherChunk ch = chunks[blockIdx.x];
+    const GatherBin b = bins[ch.bin];
+    int gy[kGatherPer], gx[kGatherPer];
+    float2 acc[kGatherPer];
+#pragma unroll
+    for (int k = 0; k < kGatherPer; k++) {
+        const int p = threadIdx.x + 256 * k;
+        gy[k] = b.ty * kGatherTile + p / kGatherTile;
+        gx[k] = b.tx * kGatherTile + p % kGatherTile;
+        acc[k] = make_float2(0.f, 0.f);
+    }
+    for (int i = ch.begin; i < ch.end; i++) {
+        const int s = slots[i];
+        const SlotCorner c = corners[s];
+        const float2* sub = scratch + (size_t)s * S * S;
+#pragma unroll
+        for (int k = 0; k < kGatherPer; k++) {
+            const unsigned y = (unsigned)(gy[k] - c.y0), x = (unsigned)(gx[k] - c.x0);
+            const bool in = y < (unsigned)S && x < (unsigned)S;
+            const float2 v = sub[in ? y * S + x : 0];
+            if (in) {
+                acc[k].x += v.x;
+                acc[k].y += v.y;
+            }
+        }
+    }
+    float2* out = partial + (size_t)blockIdx.x * kGatherTile * kGatherTile;
+#pragma unroll
+    for (int k = 0; k < kGatherPer; k++) out[threadIdx.x + 256 * k] = acc[k];
+}
+
+// Pass 2: one workgroup per bin; each pixel sums its tile's chunk partials in
+// chunk order and is written to the (quadrant-swapped) target.
+__global__ __launch_bounds__(256) void gather_bins_kernel(const GatherBin* bins, const float2* partial, int G,
+                                                          float2* base) {
+    const GatherBin b = bins[blockIdx.x];
+    float2* layer = base + (size_t)b.layer * G * G;
+    for (int p = threadIdx.x; p < kGatherTile * kGatherTile; p += blockDim.x) {
+        const int gy = b.ty * kGatherTile + p / kGatherTile, gx = b.tx * kGatherTile + p % kGatherTile;
+        if (gy >= G || gx >= G) continue;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int c = b.chunk0; c < b.chunk0 + b.nchunk; c++) {
+            const float2 v = partial[(size_t)c * kGatherTile * kGatherTile + p];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        layer[(size_t)shifted(gy, G) * G + shifted(gx, G)] = acc;
+    }
+}
+
 inline unsigned blocks_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
@@ -227,6 +284,17 @@ hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t
 hipError_t launch_zero_boxes(float2* base, const Box* boxes, int nboxes, int G, hipStream_t st) {
     if (nboxes <= 0) return hipSuccess;
     hipLaunchKernelGGL(dev::zero_boxes_kernel, dim3(64, nboxes), dim3(256), 0, st, base, boxes, G);
+    return hipGetLastError();
+}
+hipError_t launch_gather_slots(const GatherBin* bins, int nbins, const GatherChunk* chunks, int nchunks,
+                               const int32_t* slots, const SlotCorner* corners, const float2* scratch, int S, int G,
+                               float2* partial, float2* base, hipStream_t st) {
+    if (nbins <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dev::gather_chunks_kernel, dim3(nchunks), dim3(256), 0, st, bins, chunks, slots, corners,
+                       scratch, S, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dev::gather_bins_kernel, dim3(nbins), dim3(256), 0, st, bins, partial, G, base);
     return hipGetLastError();
 }
 hipError_t launch_scale_c64(float2* p, size_t n, double inv_norm, hipStream_t st) {

@@ -46,8 +46,11 @@ bool subgrid_size_supported(int s);
 // `sidx` (k counts the subgrid's visibilities in time-major, channel-minor
 // order).  Partial subgrids of one subgrid add linearly through the subgrid
 // FFT and the adder, so large subgrids can be split across items.
+// Visibilities [k0, k1) of subgrid sidx; inst: the observation in a batched
+// queue; slot: the item's place in (subgrid, k0) order, its scratch subgrid
+// for the deterministic adder.
 struct WorkItem {
-    int32_t sidx, k0, k1, inst;
+    int32_t sidx, k0, k1, inst, slot;
 };
 
 // How a gridder / degridder launch is shaped.
@@ -60,6 +63,22 @@ struct LaunchCfg {
 // (unshifted) coordinates; zero_boxes applies the quadrant swap.
 struct Box {
     int32_t layer, y0, y1, x0, x1;
+};
+
+// Deterministic adder: a 32 x 32 tile of one layer (unshifted grid
+// coordinates) whose covering scratch slots, in ascending order, are cut
+// into chunks[chunk0, chunk0 + nchunk) of at most kGatherChunk slots each
+// (slots[begin, end)); each slot's S x S box starts at its corner.
+constexpr int kGatherTile = 32;
+constexpr int kGatherChunk = 32;
+struct GatherBin {
+    int32_t layer, ty, tx, chunk0, nchunk;
+};
+struct GatherChunk {
+    int32_t bin, begin, end;
+};
+struct SlotCorner {
+    int32_t y0, x0;
 };
 
 // Kernel launch parameters shared by host and device code.
@@ -75,6 +94,9 @@ struct KernelArgs {
     const float2* adder_phasor;  // [S][S] exp(i pi (x + y - S) / S)
     const float* rtab;     // [S][S] r(x, y) = n(l, m) - n(l, 0) - n(0, m) (MFMA kernels)
     float2* grid;          // grid or layer stack (gridder output)
+    // non-null: the adder writes each work item's normalised subgrid to
+    // scratch[slot][S][S] (subgrid order) instead of adding it onto grid
+    float2* scratch;
     const float2* model_grid;  // [G][G] (degrid input, unshifted as the caller passes it)
     int32_t G, T, C, L;
     float image_size;
@@ -135,6 +157,13 @@ hipError_t launch_image_to_grid(const float* img, int G, int shift, float2* grid
 hipError_t launch_residual(const float2* vis, const float2* model, const uint8_t* flags,
                            float2* out, size_t n, hipStream_t st);
 hipError_t launch_zero_boxes(float2* base, const Box* boxes, int nboxes, int G, hipStream_t st);
+// Writes every pixel of each bin's tile as the sum of its chunks' partial
+// sums in chunk order, each partial the sum of its slots' scratch values in
+// slot order (the gridder target is zero elsewhere).  partial: one tile per
+// chunk.
+hipError_t launch_gather_slots(const GatherBin* bins, int nbins, const GatherChunk* chunks, int nchunks,
+                               const int32_t* slots, const SlotCorner* corners, const float2* scratch, int S, int G,
+                               float2* partial, float2* base, hipStream_t st);
 hipError_t launch_sum_layers(const float2* layers, int L, size_t GG, float2* out, hipStream_t st);
 hipError_t launch_sum_layer_boxes(const float2* layers, const Box* boxes, int nboxes, int G, Box u,
                                   float2* out, hipStream_t st);

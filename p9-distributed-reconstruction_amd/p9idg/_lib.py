@@ -96,6 +96,7 @@ IDG_SIGNATURES = {
     "p9idg_get_constants": [_vp, _vp, ctypes.POINTER(_i32)],
     "p9idg_destroy": [_vp],
     "p9idg_set_kernel": [_vp, _i32],
+    "p9idg_set_deterministic": [_vp, _i32],
     "p9idg_get_kernel": [_vp, ctypes.POINTER(_i32)],
     "p9idg_set_observation": [_vp, _vp, _i32, _i32, _vp, _i32, _vp, _i64],
     "p9idg_set_launch": [_vp, _i32, _i32, _i32, _i32],

@@ -185,6 +185,10 @@ class Plan:
         k = _KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
         check(_lib.idg().p9idg_set_kernel(self._h, k))
 
+    def set_deterministic(self, on=True):
+        """Ordered subgrid sums: gridding results identical from run to run."""
+        check(_lib.idg().p9idg_set_deterministic(self._h, 1 if on else 0))
+
     def set_launch(self, queue=True, grid_chunk=2048, degrid_chunk=2048, workgroups=0):
         check(_lib.idg().p9idg_set_launch(self._h, 1 if queue else 0, int(grid_chunk), int(degrid_chunk), int(workgroups)))
 

@@ -205,6 +205,78 @@ def test_reobserve_after_failed_set_observation(O):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("S,kernel", [(32, "mfma_bf16"), (24, "mfma_bf16"), (48, "mfma_fp16"), (16, "fp32")])
+def test_deterministic_adder(O, S, kernel):
+    """Ordered subgrid sums: GridW and the w-sum grid are identical bit for
+    bit across repeats and persistent-workgroup counts (same work items),
+    with and without the queue, equal the oracle, and equal the atomic adder
+    to float rounding; negative-w mirrors included."""
+    ob = obs.custom(G=256, S=S, K=8, L=4, n_ant=10, T=40, C=6, seed=90 + S, core_diam=1500.0)
+    part = ob.partition()
+    assert (part.subgrids["WLambda"] < 0).any() and (part.subgrids["WLambda"] >= 0).any()
+    oc = oracle_constants(ob.c)
+    ref_w = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.set_launch(queue=True, grid_chunk=96, degrid_chunk=96, workgroups=0)
+        atomic = plan.gridw(ob.vis)
+        plan.set_deterministic(True)
+        runs = []
+        for wg in (0, 5, 0):
+            plan.set_launch(queue=True, grid_chunk=96, degrid_chunk=96, workgroups=wg)
+            runs.append(plan.gridw(ob.vis))
+        for r in runs[1:]:
+            assert np.array_equal(r, runs[0])
+        assert rel_err(runs[0], ref_w) <= TOL
+        assert rel_err(runs[0], atomic) <= 1e-6
+        # one workgroup per subgrid (slot = subgrid), then whole-subgrid items
+        plan.set_launch(queue=False)
+        one = plan.gridw(ob.vis)
+        assert rel_err(one, ref_w) <= TOL
+        plan.set_launch(queue=True, grid_chunk=0, degrid_chunk=0, workgroups=3)
+        assert np.array_equal(plan.gridw(ob.vis), one)
+        # the w-sum grid: all layers onto one grid
+        plan.upload_vis(ob.vis)
+        plan.exec_grid(p9idg.GRID_WSUM)
+        wsum = plan.download_grid(p9idg.GRID_WSUM)
+        plan.exec_grid(p9idg.GRID_WSUM)
+        assert np.array_equal(plan.download_grid(p9idg.GRID_WSUM), wsum)
+        assert rel_err(wsum, ref_w.sum(axis=0)) <= TOL
+
+
+def test_deterministic_adder_plain_grid_and_graph(small, small_ref, O):
+    """Deterministic Grid (Adder.cs) and a replayed major-cycle graph: every
+    replay equals the eager step bit for bit."""
+    ob, part, oc = small
+    G = ob.c.GridSize
+    with p9idg.Plan(ob.c, kernel="mfma_bf16") as plan:
+        plan.set_deterministic(True)
+        plan.set_observation(ob.uvw, ob.freq, part)
+        g1 = plan.grid(ob.vis)
+        assert np.array_equal(plan.grid(ob.vis), g1)
+        assert rel_err(g1, small_ref["grid"]) <= TOL
+        model = np.zeros((G, G), np.float32)
+        model[G // 2 - 12, G // 2 + 20] = 0.7
+        plan.upload_vis(ob.vis)
+        plan.upload_image(model)
+        plan.exec_cycle(p9idg.GRID_WSTACK)
+        eager = plan.download_image()
+        plan.upload_image(model)
+        plan.capture_cycle(p9idg.GRID_WSTACK)
+        for rep in range(3):
+            plan.upload_image(model)
+            plan.exec_grid(p9idg.GRID_PLAIN)  # eager work between replays
+            plan.sync()
+            plan.upload_image(model)
+            plan.launch_cycle()
+            plan.sync()
+            assert np.array_equal(plan.download_image(), eager), rep
+        plan.set_launch(queue=True, grid_chunk=128, degrid_chunk=128)  # new items drop the graph
+        with pytest.raises(p9idg.P9IDGError) as e:
+            plan.launch_cycle()
+        assert e.value.status == p9idg._lib.ERR_STATE
+
+
 def test_bad_metadata_rejected():
     ob = obs.custom(G=128, S=16, K=4, L=2, n_ant=6, T=10, C=3, seed=6)
     part = ob.partition()
