@@ -14,6 +14,7 @@ host cores on a stride sample of the subgrids; the same sample gives the
 error of the GPU result against the oracle.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -219,6 +220,8 @@ def run_cycle(args):
                                                        s["imaging_ms"]))
     total_iters = int(stats["iterations"].sum())
     cd_ms = float(stats["deconv_ms"].sum())
+    model_digest = hashlib.sha1(np.ascontiguousarray(x, dtype=np.float32).tobytes()).hexdigest()[:16]
+    log("model image: sha1 %s, sum %.9g, max %.9g" % (model_digest, float(np.sum(x, dtype=np.float64)), float(x.max())))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -286,6 +289,7 @@ def run_cycle(args):
             "workload": "large + major cycles: psf cut 16, lambda 1.0, alpha 0.01, 30000 iterations, epsilon 1e-5",
             "kernel": args.kernel,
             "deterministic_adder": bool(args.deterministic),
+            "model_sha1": model_digest,
             "cycles_run": len(stats),
             "deconvolution_iterations": total_iters,
             "deconvolution_ms": cd_ms,
