@@ -432,9 +432,11 @@ def test_major_cycle_step_eager_graph_and_oracle(small, O, kernel):
             assert rel_err(img, eager) <= 1e-6, rep
 
 
-def test_batched_uneven_instances_match_oracle(O):
+@pytest.mark.parametrize("kernels", [("mfma_bf16",), ("fp32",), ("mfma_fp16", "fp32")])
+def test_batched_uneven_instances_match_oracle(O, kernels):
     """Observations with subgrid sizes 24, 32 and 48 in one batch: one
-    persistent launch per size class, eager and replayed from a hipGraph."""
+    persistent launch per (size, kernel) group, eager and replayed from a
+    hipGraph; the plans' kernels taken in turn from `kernels`."""
     obsv = [
         obs.custom(G=256, S=24, K=8, L=4, n_ant=9, T=30, C=5, seed=61),
         obs.custom(G=512, S=32, K=8, L=4, n_ant=12, T=60, C=8, seed=62),
@@ -449,7 +451,7 @@ def test_batched_uneven_instances_match_oracle(O):
             ref_l = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
             model = ref_l.sum(axis=0)
             ref_v = O.degridw(oc, part.subgrids, model, ob.uvw, ob.freq)
-            p = p9idg.Plan(ob.c, kernel="mfma_bf16")
+            p = p9idg.Plan(ob.c, kernel=kernels[len(plans) % len(kernels)])
             p.set_launch(queue=True, grid_chunk=128, degrid_chunk=160)
             p.set_observation(ob.uvw, ob.freq, part)
             p.upload_vis(ob.vis)
@@ -496,6 +498,11 @@ def test_batched_uneven_instances_match_oracle(O):
             assert rel_err(plans[0].download_model_vis(), ref_v) <= TOL
             for p, (ref_l, ref_v) in zip(plans[1:], refs[1:]):
                 assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
+            # batches keep the atomic adder: a deterministic member is refused
+            plans[1].set_deterministic(True)
+            with pytest.raises(p9idg.P9IDGError) as e:
+                batch.prepare()
+            assert e.value.status == p9idg._lib.ERR_UNSUPPORTED
     finally:
         for p in plans:
             p.close()
