@@ -375,6 +375,7 @@ void drop_graph(p9idg_plan* p);
 
 int upload_items(p9idg_plan* p) {
     DeviceGuard dg(p->device);
+    P9_HIP(hipStreamSynchronize(p->stream));  // queued launches may still read the old arrays
     drop_graph(p);  // a captured step holds the old item arrays
     p->gather_valid = false;
     dev_free(p->d_items_grid);
@@ -440,17 +441,20 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
 // order, the slots whose box (the adder's corner after the negative-w
 // mirror; the adder's skips excluded) overlaps it.
 int prepare_gather(p9idg_plan* p, int mode) {
+    // valid until the work items change (upload_items) or the mode does
+    if (p->gather_valid && p->gather_mode == mode) return P9IDG_OK;
     const int S = p->c.subgrid_size, G = p->c.grid_size, L = p->c.wlayer_count;
     const std::vector<WorkItem> items =
         make_items(p->meta_host, p->order_host, p->C, p->use_queue ? p->grid_chunk : 0);
     const size_t need = items.size() * (size_t)S * S;
+    P9_HIP(hipStreamSynchronize(p->stream));  // queued launches may still use the old buffers
+    p->gather_valid = false;
     if (p->alloc_scratch != need) {
         dev_free(p->d_scratch);
         p->alloc_scratch = 0;
         if (int st = dev_alloc(&p->d_scratch, std::max<size_t>(need, 1))) return st;
         p->alloc_scratch = need;
     }
-    if (p->gather_valid && p->gather_mode == mode) return P9IDG_OK;
     const int nt = (G + kGatherTile - 1) / kGatherTile, nl = mode == P9IDG_GRID_WSTACK ? L : 1;
     std::vector<SlotCorner> corner(items.size());
     std::vector<int32_t> layer(items.size(), -1);  // -1: the adder skips the slot
@@ -952,6 +956,7 @@ int p9idg_set_observation(p9idg_plan* p, const double* uvw, int32_t B, int32_t T
         gridded += (int64_t)m.time_sample_count * C;
     }
     DeviceGuard dg(p->device);
+    P9_HIP(hipStreamSynchronize(p->stream));  // queued launches may still read the buffers freed below
     p->obs_gen++;
     // drop every buffer whose allocated size differs from the new shape (by
     // the recorded sizes, so a call that failed part-way leaves nothing
