@@ -142,3 +142,46 @@ def test_gridding_constants_mirror():
     assert c.WStepLambda == 0 and c.SubgridSpheroidal.shape == (32, 32) and np.all(c.SubgridSpheroidal == 1)
     s = c._struct()
     assert (s.grid_size, s.subgrid_size, s.kernel_size, s.wlayer_count, s.visibilities_count) == (3072, 32, 16, 24, 12345)
+
+
+def handle_functions():
+    """(header, name, parameter count) of every entry point whose first
+    parameter is a plan, batch, convolver or deconvolver handle."""
+    out = []
+    for header in ("p9idg.h", "p9cycle.h"):
+        text = open(os.path.join(ROOT, "include", header)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\bint\s+(p9\w+)\s*\(([^;]*?)\)\s*;", text, flags=re.S):
+            params = [p.strip() for p in m.group(2).split(",") if p.strip()]
+            if params and re.match(r"(const\s+)?p9(idg_plan|idg_batch|conv|cd)\s*\*\s*\w+$", params[0]):
+                out.append((header, m.group(1), len(params)))
+    return out
+
+
+def test_null_handles_are_rejected():
+    """Every entry point taking a handle returns P9IDG_ERR_ARGUMENT for NULL
+    (destroy: OK) before touching the GPU, whatever its other arguments."""
+    lib = _lib.idg()
+    funcs = handle_functions()
+    assert len(funcs) >= 60
+    for header, name, n in funcs:
+        f = getattr(lib, name)
+        assert len(f.argtypes) == n, name
+        pointer = lambda t: t in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(t, ctypes._Pointer)
+        st = f(*[None if pointer(t) else 0 for t in f.argtypes])
+        if name.endswith("_destroy"):
+            assert st == _lib.OK, name
+        else:
+            assert st == _lib.ERR_ARGUMENT, (name, st)
+
+
+def test_fits_null_arguments_are_rejected():
+    """The FITS readers and writers return P9IDG_ERR_ARGUMENT for NULL paths
+    and buffers instead of dereferencing them."""
+    lib = _lib.idg()
+    pointer = lambda t: t in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(t, ctypes._Pointer)
+    names = [n for n in declared("p9fits.h", "p9fits_") if n != "p9fits_set_threads"]
+    assert len(names) == 9
+    for name in names:
+        f = getattr(lib, name)
+        assert f(*[None if pointer(t) else 0 for t in f.argtypes]) == _lib.ERR_ARGUMENT, name
