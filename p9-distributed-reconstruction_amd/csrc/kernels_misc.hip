@@ -154,8 +154,17 @@ constexpr int kGatherPer = kGatherTile * kGatherTile / 256;
 __global__ __launch_bounds__(256) void gather_chunks_kernel(const GatherBin* bins, const GatherChunk* chunks,
                                                             const int32_t* slots, const SlotCorner* corners,
                                                             const float2* scratch, int S, float2* partial) {
+    __shared__ int32_t s_slot[kGatherChunk];
+    __shared__ SlotCorner s_corner[kGatherChunk];
     const GatherChunk ch = chunks[blockIdx.x];
     const GatherBin b = bins[ch.bin];
+    const int n = ch.end - ch.begin;  // <= kGatherChunk
+    if ((int)threadIdx.x < n) {
+        const int s = slots[ch.begin + threadIdx.x];
+        s_slot[threadIdx.x] = s;
+        s_corner[threadIdx.x] = corners[s];
+    }
+    __syncthreads();
     int gy[kGatherPer], gx[kGatherPer];
     float2 acc[kGatherPer];
 #pragma unroll
@@ -165,9 +174,10 @@ __global__ __launch_bounds__(256) void gather_chunks_kernel(const GatherBin* bin
         gx[k] = b.tx * kGatherTile + p % kGatherTile;
         acc[k] = make_float2(0.f, 0.f);
     }
-    for (int i = ch.begin; i < ch.end; i++) {
-        const int s = slots[i];
-        const SlotCorner c = corners[s];
+#pragma unroll 8
+    for (int i = 0; i < n; i++) {
+        const int s = s_slot[i];
+        const SlotCorner c = s_corner[i];
         const float2* sub = scratch + (size_t)s * S * S;
 #pragma unroll
         for (int k = 0; k < kGatherPer; k++) {
@@ -195,6 +205,7 @@ __global__ __launch_bounds__(256) void gather_bins_kernel(const GatherBin* bins,
         const int gy = b.ty * kGatherTile + p / kGatherTile, gx = b.tx * kGatherTile + p % kGatherTile;
         if (gy >= G || gx >= G) continue;
         float2 acc = make_float2(0.f, 0.f);
+#pragma unroll 8
         for (int c = b.chunk0; c < b.chunk0 + b.nchunk; c++) {
             const float2 v = partial[(size_t)c * kGatherTile * kGatherTile + p];
             acc.x += v.x;

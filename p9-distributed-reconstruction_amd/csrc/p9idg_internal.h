@@ -70,7 +70,7 @@ struct Box {
 // into chunks[chunk0, chunk0 + nchunk) of at most kGatherChunk slots each
 // (slots[begin, end)); each slot's S x S box starts at its corner.
 constexpr int kGatherTile = 32;
-constexpr int kGatherChunk = 32;
+constexpr int kGatherChunk = 16;
 struct GatherBin {
     int32_t layer, ty, tx, chunk0, nchunk;
 };
