@@ -618,6 +618,20 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
     }
     const float out_scale = a.degrid_scale / pscale;
     const float q1_back = kTwoPiF / rscale;
+    // Up to S = 32 the products' minus signs sit on loop-invariant copies of
+    // -Pr and -rPr, so no tile negates its x-phasors; S = 48 keeps the
+    // registers and negates the phasors instead.
+    constexpr bool kNegP = RT * KS <= 2;
+    u32x4 nPrh[RT][KS], nPrl[RT][KS], nQr[RT][KS];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++)
+            if constexpr (kNegP) {
+                nPrh[rt][ks] = neg(Prh[rt][ks]);
+                nPrl[rt][ks] = neg(Prl[rt][ks]);
+                nQr[rt][ks] = neg(Qr[rt][ks]);
+            }
 
     // One tile: Q0 = P conj(A), Q1 = (r P) conj(A) on the MFMA, then
     // vis = sum_y conj(B_v[y]) (Q0 - 2 pi i c Q1) over this lane's rows and
@@ -634,26 +648,48 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 P9_SPLIT2(P, sk, A[8 * ks + 2 * q].x, A[8 * ks + 2 * q + 1].x, Arh[q], Arl[q]);
                 P9_SPLIT2(P, sk, A[8 * ks + 2 * q].y, A[8 * ks + 2 * q + 1].y, Aih[q], Ail[q]);
             }
-            const u32x4 nAih = neg(Aih), nAil = neg(Ail);
+            // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar - Pr Ai (hi*hi + hi*lo + lo*hi)
+            if constexpr (kNegP) {
 #pragma unroll
-            for (int rt = 0; rt < RT; rt++) {
-                // Re Q0 = Pr Ar + Pi Ai, Im Q0 = Pi Ar + Pr (-Ai) (hi*hi + hi*lo + lo*hi)
-                q0r[rt] = P::mma16(Prh[rt][ks], Arh, q0r[rt]);
-                q0i[rt] = P::mma16(Pih[rt][ks], Arh, q0i[rt]);
-                q1r[rt] = P::mma16(Qr[rt][ks], Arh, q1r[rt]);
-                q1i[rt] = P::mma16(Qi[rt][ks], Arh, q1i[rt]);
-                q0r[rt] = P::mma16(Prh[rt][ks], Arl, q0r[rt]);
-                q0i[rt] = P::mma16(Pih[rt][ks], Arl, q0i[rt]);
-                q1r[rt] = P::mma16(Qi[rt][ks], Aih, q1r[rt]);
-                q1i[rt] = P::mma16(Qr[rt][ks], nAih, q1i[rt]);
-                q0r[rt] = P::mma16(Prl[rt][ks], Arh, q0r[rt]);
-                q0i[rt] = P::mma16(Pil[rt][ks], Arh, q0i[rt]);
-                q0r[rt] = P::mma16(Pih[rt][ks], Aih, q0r[rt]);
-                q0i[rt] = P::mma16(Prh[rt][ks], nAih, q0i[rt]);
-                q0r[rt] = P::mma16(Pih[rt][ks], Ail, q0r[rt]);
-                q0i[rt] = P::mma16(Prh[rt][ks], nAil, q0i[rt]);
-                q0r[rt] = P::mma16(Pil[rt][ks], Aih, q0r[rt]);
-                q0i[rt] = P::mma16(Prl[rt][ks], nAih, q0i[rt]);
+                for (int rt = 0; rt < RT; rt++) {
+                    q0r[rt] = P::mma16(Prh[rt][ks], Arh, q0r[rt]);
+                    q0i[rt] = P::mma16(Pih[rt][ks], Arh, q0i[rt]);
+                    q1r[rt] = P::mma16(Qr[rt][ks], Arh, q1r[rt]);
+                    q1i[rt] = P::mma16(Qi[rt][ks], Arh, q1i[rt]);
+                    q0r[rt] = P::mma16(Prh[rt][ks], Arl, q0r[rt]);
+                    q0i[rt] = P::mma16(Pih[rt][ks], Arl, q0i[rt]);
+                    q1r[rt] = P::mma16(Qi[rt][ks], Aih, q1r[rt]);
+                    q1i[rt] = P::mma16(nQr[rt][ks], Aih, q1i[rt]);
+                    q0r[rt] = P::mma16(Prl[rt][ks], Arh, q0r[rt]);
+                    q0i[rt] = P::mma16(Pil[rt][ks], Arh, q0i[rt]);
+                    q0r[rt] = P::mma16(Pih[rt][ks], Aih, q0r[rt]);
+                    q0i[rt] = P::mma16(nPrh[rt][ks], Aih, q0i[rt]);
+                    q0r[rt] = P::mma16(Pih[rt][ks], Ail, q0r[rt]);
+                    q0i[rt] = P::mma16(nPrh[rt][ks], Ail, q0i[rt]);
+                    q0r[rt] = P::mma16(Pil[rt][ks], Aih, q0r[rt]);
+                    q0i[rt] = P::mma16(nPrl[rt][ks], Aih, q0i[rt]);
+                }
+            } else {
+                const u32x4 nAih = neg(Aih), nAil = neg(Ail);
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) {
+                    q0r[rt] = P::mma16(Prh[rt][ks], Arh, q0r[rt]);
+                    q0i[rt] = P::mma16(Pih[rt][ks], Arh, q0i[rt]);
+                    q1r[rt] = P::mma16(Qr[rt][ks], Arh, q1r[rt]);
+                    q1i[rt] = P::mma16(Qi[rt][ks], Arh, q1i[rt]);
+                    q0r[rt] = P::mma16(Prh[rt][ks], Arl, q0r[rt]);
+                    q0i[rt] = P::mma16(Pih[rt][ks], Arl, q0i[rt]);
+                    q1r[rt] = P::mma16(Qi[rt][ks], Aih, q1r[rt]);
+                    q1i[rt] = P::mma16(Qr[rt][ks], nAih, q1i[rt]);
+                    q0r[rt] = P::mma16(Prl[rt][ks], Arh, q0r[rt]);
+                    q0i[rt] = P::mma16(Pil[rt][ks], Arh, q0i[rt]);
+                    q0r[rt] = P::mma16(Pih[rt][ks], Aih, q0r[rt]);
+                    q0i[rt] = P::mma16(Prh[rt][ks], nAih, q0i[rt]);
+                    q0r[rt] = P::mma16(Pih[rt][ks], Ail, q0r[rt]);
+                    q0i[rt] = P::mma16(Prh[rt][ks], nAil, q0i[rt]);
+                    q0r[rt] = P::mma16(Pil[rt][ks], Aih, q0r[rt]);
+                    q0i[rt] = P::mma16(Prl[rt][ks], nAih, q0i[rt]);
+                }
             }
         }
         float sr = 0.f, si = 0.f;
