@@ -453,8 +453,9 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
         const int nsteps = (min(ROUND, K - j0) + 31) / 32;
         for (int st = wave; st < nsteps; st += 4) {
             const int jg = st * 32 + grp * 8;  // this lane group's 8 visibilities
-            // x-phasors of the column tiles, split
-            u32x4 Arh[NT], Arl[NT], Aih[NT], Ail[NT];
+            // x-phasors of the column tiles, split; -Ai carries the products'
+            // minus signs (one negation per column tile, not three per row tile)
+            u32x4 Arh[NT], Arl[NT], Aih[NT], Ail[NT], nAih[NT], nAil[NT];
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 float2 A[8];
@@ -464,6 +465,8 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
                     P9_SPLIT2(P, sk, A[2 * q].x, A[2 * q + 1].x, Arh[t][q], Arl[t][q]);
                     P9_SPLIT2(P, sk, A[2 * q].y, A[2 * q + 1].y, Aih[t][q], Ail[t][q]);
                 }
+                nAih[t] = neg(Aih[t]);
+                nAil[t] = neg(Ail[t]);
             }
             // c of the 8 visibilities (FP32; scaled for FP16)
             float cf[8];
@@ -482,7 +485,6 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
                     P9_SPLIT2(P, sk, L0.x, L1.x, Lrh[q], Lrl[q]);
                     P9_SPLIT2(P, sk, L0.y, L1.y, Lih[q], Lil[q]);
                 }
-                const u32x4 nLih = neg(Lih), nLil = neg(Lil), ncLi = neg(cLi);
 #pragma unroll
                 for (int q = 0; q < NT; q++) {
                     // Re M0 = Lr Ar - Li Ai, Im M0 = Lr Ai + Li Ar (hi*hi + hi*lo + lo*hi)
@@ -494,15 +496,15 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
                     acc1i[r][q] = P::mma16(cLr, Aih[q], acc1i[r][q]);
                     m0r = P::mma16(Lrh, Arl[q], m0r);
                     m0i = P::mma16(Lrh, Ail[q], m0i);
-                    acc1r[r][q] = P::mma16(ncLi, Aih[q], acc1r[r][q]);
+                    acc1r[r][q] = P::mma16(cLi, nAih[q], acc1r[r][q]);
                     acc1i[r][q] = P::mma16(cLi, Arh[q], acc1i[r][q]);
                     m0r = P::mma16(Lrl, Arh[q], m0r);
                     m0i = P::mma16(Lrl, Aih[q], m0i);
-                    m0r = P::mma16(nLih, Aih[q], m0r);
+                    m0r = P::mma16(Lih, nAih[q], m0r);
                     m0i = P::mma16(Lih, Arh[q], m0i);
-                    m0r = P::mma16(nLih, Ail[q], m0r);
+                    m0r = P::mma16(Lih, nAil[q], m0r);
                     m0i = P::mma16(Lih, Arl[q], m0i);
-                    m0r = P::mma16(nLil, Aih[q], m0r);
+                    m0r = P::mma16(Lil, nAih[q], m0r);
                     m0i = P::mma16(Lil, Arh[q], m0i);
                 }
             }
