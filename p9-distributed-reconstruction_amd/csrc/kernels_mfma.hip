@@ -189,7 +189,7 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 
     // FP16: scale the visibilities so |V| <= 1, and c * L by 2^-16.
     float vscale = 1.f;
-    float cscale = 1.f;
+    constexpr float cscale = P::kNormalise ? 1.f / 65536.f : 1.f;  // BF16: no multiply
     if (P::kNormalise) {
         float m = 0.f;
         for (int j = k0 + tid; j < K; j += kThreads) {
@@ -197,7 +197,6 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
             m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
         }
         vscale = 1.f / pow2_ceil(block_max(m, s_red));
-        cscale = 1.f / 65536.f;
     }
 
     // this wave's tile and the lane's row y (A operand) and column x (B operand)
@@ -378,7 +377,7 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
 
     // FP16: scale the visibilities so |V| <= 1, and c * L by 2^-16.
     float vscale = 1.f;
-    float cscale = 1.f;
+    constexpr float cscale = P::kNormalise ? 1.f / 65536.f : 1.f;  // BF16: no multiply
     if (P::kNormalise) {
         float m = 0.f;
         for (int j = k0 + tid; j < K; j += kThreads) {
@@ -386,7 +385,6 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
             m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
         }
         vscale = 1.f / pow2_ceil(block_max(m, s_red));
-        cscale = 1.f / 65536.f;
     }
 
     // l and the separable n of coordinate 16 t + c (rows and columns alike)
