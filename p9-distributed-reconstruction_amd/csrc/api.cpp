@@ -1558,9 +1558,12 @@ void batch_free_groups(p9idg_batch* b) {
 // work starts after it.
 int batch_check(const p9idg_batch* b) {
     if (!b->prepared) return fail(P9IDG_ERR_STATE, "batch not prepared (p9idg_batch_prepare)");
-    for (size_t i = 0; i < b->plans.size(); i++)
+    for (size_t i = 0; i < b->plans.size(); i++) {
         if (b->plans[i]->obs_gen != b->obs_gen[i])
             return fail(P9IDG_ERR_STATE, "a plan's observation changed after p9idg_batch_prepare");
+        if (b->plans[i]->deterministic)
+            return fail(P9IDG_ERR_UNSUPPORTED, "the deterministic adder is not available in batches");
+    }
     return P9IDG_OK;
 }
 int batch_order_in(p9idg_batch* b) {

@@ -498,11 +498,13 @@ def test_batched_uneven_instances_match_oracle(O, kernels):
             assert rel_err(plans[0].download_model_vis(), ref_v) <= TOL
             for p, (ref_l, ref_v) in zip(plans[1:], refs[1:]):
                 assert rel_err(p.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
-            # batches keep the atomic adder: a deterministic member is refused
+            # batches keep the atomic adder: a deterministic member is refused,
+            # also when it turns deterministic after the batch was prepared
             plans[1].set_deterministic(True)
-            with pytest.raises(p9idg.P9IDGError) as e:
-                batch.prepare()
-            assert e.value.status == p9idg._lib.ERR_UNSUPPORTED
+            for call in (lambda: batch.exec(3), batch.prepare):
+                with pytest.raises(p9idg.P9IDGError) as e:
+                    call()
+                assert e.value.status == p9idg._lib.ERR_UNSUPPORTED
     finally:
         for p in plans:
             p.close()
