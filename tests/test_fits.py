@@ -14,6 +14,7 @@ package, which is not in its tree, so the readers are pinned three ways:
 
 These run on the CPU: the FITS code is host code in libp9idg.so.
 """
+import ctypes
 import os
 import struct
 
@@ -307,3 +308,56 @@ def test_simulated_points_round_trip_of_an_observation(tmp_path):
     c = MeasurementData.CreateSimulatedStandardParams(d.VisibilitiesCount)
     assert (c.GridSize, c.SubgridSize, c.KernelSize, c.MaxTimestepsPerSubgrid, c.WLayerCount) == (256, 8, 4, 1024, 1)
     assert c.ScaleArcSec == np.float32(1.0 / 3600.0 * np.pi / 180.0)
+
+
+def test_corrupted_files_fail_cleanly(tmp_path):
+    """Truncated files, flipped header bytes, absurd or non-numeric axis
+    values and random bytes: every reader returns OK, P9IDG_ERR_IO or
+    P9IDG_ERR_ARGUMENT (never reads past the file or writes past the
+    caller's buffer, whose tail is guarded)."""
+    lib = _lib.idg()
+    rng = np.random.default_rng(2024)
+    valid = [OF.encode(np.arange(12, dtype=np.float64), -64),
+             OF.encode(rng.standard_normal((3, 4)).astype(np.float32), -32),
+             OF.encode(rng.standard_normal((2, 5, 3)), -64),
+             OF.encode(rng.standard_normal((1, 1, 3, 4)).astype(np.float32), -32)]
+    bad_values = [b"-1", b"0", b"999999999", b"2147483647", b"-2147483648", b"1e300", b"abc", b"", b"T", b"9" * 40]
+    ok = {_lib.OK, _lib.ERR_IO, _lib.ERR_ARGUMENT}
+    guard = 1e30
+    for trial in range(400):
+        data = bytearray(valid[trial % len(valid)])
+        kind = trial % 5
+        if kind == 0:  # truncation
+            data = data[:int(rng.integers(0, len(data)))]
+        elif kind == 1:  # random bytes flipped in the header record
+            for _ in range(int(rng.integers(1, 20))):
+                data[int(rng.integers(0, 2880))] = int(rng.integers(0, 256))
+        elif kind == 2:  # an axis or BITPIX value replaced
+            card = int(rng.integers(1, 6))
+            v = bad_values[int(rng.integers(0, len(bad_values)))]
+            field = v.rjust(20)[:70]
+            data[card * 80 + 10:card * 80 + 10 + len(field)] = field
+        elif kind == 3:  # END removed
+            i = bytes(data).find(b"END" + b" " * 77)
+            if i >= 0:
+                data[i:i + 3] = b"XYZ"
+        else:  # random bytes
+            data = bytearray(rng.integers(0, 256, int(rng.integers(0, 6000)), dtype=np.uint8).tobytes())
+        path = str(tmp_path / ("f%d.fits" % trial))
+        with open(path, "wb") as f:
+            f.write(bytes(data))
+        p = path.encode()
+        info = _lib.FitsInfo()
+        assert lib.p9fits_read_info(p, ctypes.byref(info)) in ok, trial
+        freq = np.full(17, guard)
+        n = ctypes.c_int64()
+        assert lib.p9fits_read_frequencies(p, freq.ctypes.data, 16, ctypes.byref(n)) in ok, trial
+        assert freq[16] == guard
+        b, t = ctypes.c_int32(), ctypes.c_int32()
+        assert lib.p9fits_count_baselines(p, ctypes.byref(b), ctypes.byref(t)) in ok, trial
+        img = np.full(13, guard, dtype=np.float64)
+        assert lib.p9fits_read_image(p, _lib.FITS_F64, img.ctypes.data, 3, 4) in ok, trial
+        assert img[12] == guard
+        img32 = np.full(13, guard, dtype=np.float32)
+        assert lib.p9fits_read_casa(p, img32.ctypes.data, 3, 4) in ok, trial
+        assert img32[12] == np.float32(guard)
