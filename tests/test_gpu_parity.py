@@ -126,6 +126,26 @@ def test_subgrid_sizes_and_wstack(O, S, K, L, wst, n_ant, T, C, kernel):
     assert e_g <= TOL and e_d <= TOL
 
 
+@pytest.mark.parametrize("G,S", [(250, 32), (202, 24)])
+@pytest.mark.parametrize("kernel", ["mfma_bf16", "fp32"])
+def test_grid_size_not_a_multiple_of_32(O, G, S, kernel):
+    """An even grid size that is no multiple of 32 (the deterministic sums'
+    tile size): GridW / DeGridW against the oracle, atomic and ordered."""
+    ob = obs.custom(G=G, S=S, K=8, L=4, n_ant=10, T=40, C=6, seed=G + S, core_diam=1500.0)
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    ref_l = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
+    ref_v = O.degridw(oc, part.subgrids, ref_l.sum(axis=0), ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        assert rel_err(plan.gridw(ob.vis), ref_l) <= TOL
+        assert rel_err(plan.degridw(ref_l.sum(axis=0)), ref_v) <= TOL
+        plan.set_deterministic(True)
+        det = plan.gridw(ob.vis)
+        assert rel_err(det, ref_l) <= TOL
+        assert np.array_equal(plan.gridw(ob.vis), det)
+
+
 REC_CASES = [
     # subgrid, channels, frequency jitter (Hz), work-item size: with even
     # spacing and C % 8 == 0 the gridder steps its phasors channel to
