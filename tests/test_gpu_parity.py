@@ -146,6 +146,23 @@ def test_grid_size_not_a_multiple_of_32(O, G, S, kernel):
         assert np.array_equal(plan.gridw(ob.vis), det)
 
 
+@pytest.mark.parametrize("n_ant,C", [(10, 1), (2, 8), (2, 1)])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_single_channel_and_single_baseline(O, n_ant, C, kernel):
+    """One channel (no channel stepping: C < 2), one baseline (two antennas),
+    and both at once."""
+    ob = obs.custom(G=256, S=32, K=8, L=2, n_ant=n_ant, T=60, C=C, seed=300 + n_ant + C, core_diam=1500.0)
+    part = ob.partition()
+    assert part.total > 0
+    oc = oracle_constants(ob.c)
+    ref_l = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
+    ref_v = O.degridw(oc, part.subgrids, ref_l.sum(axis=0), ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        assert rel_err(plan.gridw(ob.vis), ref_l) <= TOL
+        assert rel_err(plan.degridw(ref_l.sum(axis=0)), ref_v) <= TOL
+
+
 REC_CASES = [
     # subgrid, channels, frequency jitter (Hz), work-item size: with even
     # spacing and C % 8 == 0 the gridder steps its phasors channel to
