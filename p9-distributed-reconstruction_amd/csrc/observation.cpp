@@ -115,6 +115,8 @@ void p9obs_sky_vis(int32_t B, int32_t T, int32_t C, const double* uvw, const dou
 #ifdef _OPENMP
     const int nt = threads > 0 ? threads : omp_get_max_threads();
 #pragma omp parallel num_threads(nt)
+#else
+    (void)threads;  // one thread without OpenMP
 #endif
     {
         std::vector<double> er(np), ei(np), rr(np), ri(np), ang(np);
