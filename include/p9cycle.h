@@ -95,7 +95,10 @@ typedef struct p9cycle_stats {
 /* MajorCycle.ReconstructSerialCD (MajorCycle.cs:33-134) on a plan with its
  * observation set: PSF from the flags, then up to max_major_cycles cycles of
  * GridW -> ToImage -> gradients -> deconvolution -> ToUVW -> DeGridW ->
- * residual, all in HBM.  vis (type P9IDG_C64 / P9IDG_C128, [B][T][C]) is
+ * residual, all in HBM.  The device step ToUVW -> DeGridW -> residual ->
+ * GridW -> ToImage is recorded once as the plan's cycle graph
+ * (p9idg_capture_cycle, which also runs it) and replayed in every later
+ * cycle (p9idg_launch_cycle).  vis (type P9IDG_C64 / P9IDG_C128, [B][T][C]) is
  * uploaded after the PSF is gridded; flags (uint8 [B][T][C], NULL = none).
  * x_image receives the model image float[G][G]; stats[max_major_cycles] (may
  * be NULL) and *ncycles the cycles run.  The deconvolver is FastSerialCD's

@@ -118,14 +118,18 @@ struct p9idg_plan {
     bool deterministic = false;
     float2* d_scratch = nullptr;
     size_t alloc_scratch = 0;  // float2 elements
-    bool gather_valid = false;
-    int32_t gather_mode = -1;
-    GatherBin* d_bins = nullptr;
-    GatherChunk* d_chunks = nullptr;
-    int32_t* d_bin_slots = nullptr;
-    SlotCorner* d_corners = nullptr;
-    float2* d_partial = nullptr;  // one 32 x 32 tile per chunk
-    int32_t n_bins = 0, n_chunks = 0, n_slots = 0;
+    // Tile lists of each grid mode, kept side by side: building one mode's
+    // lists never frees another's, which a captured graph may be using.
+    // They stay valid until the work items change (upload_items).
+    struct Gather {
+        bool valid = false;
+        GatherBin* d_bins = nullptr;
+        GatherChunk* d_chunks = nullptr;
+        int32_t* d_bin_slots = nullptr;
+        SlotCorner* d_corners = nullptr;
+        float2* d_partial = nullptr;  // one 32 x 32 tile per chunk
+        int32_t n_bins = 0, n_chunks = 0;
+    } gather[3];
 
     hipfftHandle fft_one = 0;
     bool have_fft_one = false;
@@ -373,11 +377,21 @@ std::vector<WorkItem> make_items(const std::vector<p9idg_subgrid>& meta,
 
 void drop_graph(p9idg_plan* p);
 
+void free_gather(p9idg_plan::Gather& g) {
+    dev_free(g.d_bins);
+    dev_free(g.d_chunks);
+    dev_free(g.d_bin_slots);
+    dev_free(g.d_corners);
+    dev_free(g.d_partial);
+    g.n_bins = g.n_chunks = 0;
+    g.valid = false;
+}
+
 int upload_items(p9idg_plan* p) {
     DeviceGuard dg(p->device);
     P9_HIP(hipStreamSynchronize(p->stream));  // queued launches may still read the old arrays
-    drop_graph(p);  // a captured step holds the old item arrays
-    p->gather_valid = false;
+    drop_graph(p);  // a captured step holds the old item arrays and tile lists
+    for (auto& g : p->gather) free_gather(g);
     dev_free(p->d_items_grid);
     dev_free(p->d_items_degrid);
     p->nitems_grid = p->nitems_degrid = 0;
@@ -439,17 +453,22 @@ KernelArgs make_args(const p9idg_plan* p, int mode) {
 // work item (the queue's items, or one per subgrid without the queue) owns
 // one S x S scratch slot; each 32 x 32 tile of the target lists, in slot
 // order, the slots whose box (the adder's corner after the negative-w
-// mirror; the adder's skips excluded) overlaps it.
+// mirror; the adder's skips excluded) overlaps it.  Each mode keeps its own
+// lists and the scratch depends on the work items only, so preparing one
+// mode frees nothing that a graph captured in another mode uses.
 int prepare_gather(p9idg_plan* p, int mode) {
-    // valid until the work items change (upload_items) or the mode does
-    if (p->gather_valid && p->gather_mode == mode) return P9IDG_OK;
+    p9idg_plan::Gather& gt = p->gather[mode];
+    if (gt.valid) return P9IDG_OK;  // valid until the work items change (upload_items)
     const int S = p->c.subgrid_size, G = p->c.grid_size, L = p->c.wlayer_count;
     const std::vector<WorkItem> items =
         make_items(p->meta_host, p->order_host, p->C, p->use_queue ? p->grid_chunk : 0);
     const size_t need = items.size() * (size_t)S * S;
     P9_HIP(hipStreamSynchronize(p->stream));  // queued launches may still use the old buffers
-    p->gather_valid = false;
     if (p->alloc_scratch != need) {
+        // only a change of the work items changes the scratch size, and that
+        // already dropped the graph; drop it here too so no graph can keep
+        // the freed scratch
+        drop_graph(p);
         dev_free(p->d_scratch);
         p->alloc_scratch = 0;
         if (int st = dev_alloc(&p->d_scratch, std::max<size_t>(need, 1))) return st;
@@ -505,29 +524,23 @@ int prepare_gather(p9idg_plan* p, int mode) {
             chunks.push_back({b, i, std::min(i + kGatherChunk, start[k + 1])});
         bins.back().nchunk = (int32_t)chunks.size() - bins.back().chunk0;
     }
-    dev_free(p->d_bins);
-    dev_free(p->d_chunks);
-    dev_free(p->d_bin_slots);
-    dev_free(p->d_corners);
-    dev_free(p->d_partial);
-    if (int st = dev_alloc(&p->d_bins, std::max<size_t>(bins.size(), 1))) return st;
-    if (int st = dev_alloc(&p->d_chunks, std::max<size_t>(chunks.size(), 1))) return st;
-    if (int st = dev_alloc(&p->d_bin_slots, std::max<size_t>(slots.size(), 1))) return st;
-    if (int st = dev_alloc(&p->d_corners, std::max<size_t>(corner.size(), 1))) return st;
-    if (int st = dev_alloc(&p->d_partial, std::max<size_t>(chunks.size(), 1) * kGatherTile * kGatherTile)) return st;
+    free_gather(gt);  // not valid here, so no graph uses these
+    if (int st = dev_alloc(&gt.d_bins, std::max<size_t>(bins.size(), 1))) return st;
+    if (int st = dev_alloc(&gt.d_chunks, std::max<size_t>(chunks.size(), 1))) return st;
+    if (int st = dev_alloc(&gt.d_bin_slots, std::max<size_t>(slots.size(), 1))) return st;
+    if (int st = dev_alloc(&gt.d_corners, std::max<size_t>(corner.size(), 1))) return st;
+    if (int st = dev_alloc(&gt.d_partial, std::max<size_t>(chunks.size(), 1) * kGatherTile * kGatherTile)) return st;
     if (!bins.empty())
-        P9_HIP(hipMemcpy(p->d_bins, bins.data(), bins.size() * sizeof(GatherBin), hipMemcpyHostToDevice));
+        P9_HIP(hipMemcpy(gt.d_bins, bins.data(), bins.size() * sizeof(GatherBin), hipMemcpyHostToDevice));
     if (!chunks.empty())
-        P9_HIP(hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(GatherChunk), hipMemcpyHostToDevice));
+        P9_HIP(hipMemcpy(gt.d_chunks, chunks.data(), chunks.size() * sizeof(GatherChunk), hipMemcpyHostToDevice));
     if (!slots.empty())
-        P9_HIP(hipMemcpy(p->d_bin_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        P9_HIP(hipMemcpy(gt.d_bin_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     if (!corner.empty())
-        P9_HIP(hipMemcpy(p->d_corners, corner.data(), corner.size() * sizeof(SlotCorner), hipMemcpyHostToDevice));
-    p->n_bins = (int32_t)bins.size();
-    p->n_chunks = (int32_t)chunks.size();
-    p->n_slots = (int32_t)items.size();
-    p->gather_mode = mode;
-    p->gather_valid = true;
+        P9_HIP(hipMemcpy(gt.d_corners, corner.data(), corner.size() * sizeof(SlotCorner), hipMemcpyHostToDevice));
+    gt.n_bins = (int32_t)bins.size();
+    gt.n_chunks = (int32_t)chunks.size();
+    gt.valid = true;
     return P9IDG_OK;
 }
 
@@ -551,7 +564,7 @@ int do_exec_grid(p9idg_plan* p, int mode) {
     a.items = p->d_items_grid;
     a.nitems = p->nitems_grid;
     if (p->deterministic) {
-        if (p->capturing && !(p->gather_valid && p->gather_mode == mode))
+        if (p->capturing && !p->gather[mode].valid)
             return fail(P9IDG_ERR_STATE, "deterministic adder not prepared for this mode before capture");
         if (int st = prepare_gather(p, mode)) return st;
         a.scratch = p->d_scratch;
@@ -567,8 +580,9 @@ int do_exec_grid(p9idg_plan* p, int mode) {
         e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p, true), p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
     if (p->deterministic) {
-        e = launch_gather_slots(p->d_bins, p->n_bins, p->d_chunks, p->n_chunks, p->d_bin_slots, p->d_corners,
-                                p->d_scratch, p->c.subgrid_size, p->c.grid_size, p->d_partial, target, p->stream);
+        const p9idg_plan::Gather& gt = p->gather[mode];
+        e = launch_gather_slots(gt.d_bins, gt.n_bins, gt.d_chunks, gt.n_chunks, gt.d_bin_slots, gt.d_corners,
+                                p->d_scratch, p->c.subgrid_size, p->c.grid_size, gt.d_partial, target, p->stream);
         if (e != hipSuccess) return hip_fail(e, "ordered subgrid sum");
     }
     if (int st = span_end(p)) return st;
@@ -704,16 +718,31 @@ int upload_vis_any(p9idg_plan* p, const void* vis, int32_t type) {
     return P9IDG_OK;
 }
 
+// The residual kernel of a captured cycle graph holds the flag pointer it was
+// recorded with (NULL without flags); switching the flags on or off, or to a
+// new buffer, therefore drops the graph.  New contents in the same buffer are
+// read by the next replay.
+int flags_changed(p9idg_plan* p, const uint8_t* before) {
+    const uint8_t* after = p->have_flags ? p->d_flags : nullptr;
+    if (before != after && (p->graph || p->graph_exec)) {
+        P9_HIP(hipStreamSynchronize(p->stream));
+        drop_graph(p);
+    }
+    return P9IDG_OK;
+}
+
 int set_psf_vis(p9idg_plan* p, const uint8_t* flags) {
     if (int st = check_obs(p)) return st;
     DeviceGuard dg(p->device);
     const size_t n = vis_elems(p);
     const uint8_t* dflags = nullptr;
     if (flags) {
+        const uint8_t* before = p->have_flags ? p->d_flags : nullptr;
         if (int st = dev_alloc(&p->d_flags, n)) return st;
         P9_HIP(hipMemcpyAsync(p->d_flags, flags, n, hipMemcpyHostToDevice, p->stream));
         dflags = p->d_flags;
         p->have_flags = true;
+        if (int st = flags_changed(p, before)) return st;
     }
     P9_HIP(launch_psf_vis(dflags, p->d_vis, n, p->stream));
     return P9IDG_OK;
@@ -869,11 +898,7 @@ int p9idg_destroy(p9idg_plan* p) {
     dev_free(p->d_items_grid);
     dev_free(p->d_items_degrid);
     dev_free(p->d_scratch);
-    dev_free(p->d_bins);
-    dev_free(p->d_chunks);
-    dev_free(p->d_bin_slots);
-    dev_free(p->d_corners);
-    dev_free(p->d_partial);
+    for (auto& g : p->gather) free_gather(g);
     if (p->d_staging) (void)hipFree(p->d_staging);
     for (auto& ev : p->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1229,13 +1254,15 @@ int p9idg_exec_degrid(p9idg_plan* p, int32_t mode) { return do_exec_degrid(p, mo
 int p9idg_set_flags(p9idg_plan* p, const uint8_t* flags) {
     if (int st = check_obs(p)) return st;
     DeviceGuard dg(p->device);
+    const uint8_t* before = p->have_flags ? p->d_flags : nullptr;
     if (!flags) {
         p->have_flags = false;
-        return P9IDG_OK;
+        return flags_changed(p, before);
     }
     if (int st = dev_alloc(&p->d_flags, vis_elems(p))) return st;
     P9_HIP(hipMemcpyAsync(p->d_flags, flags, vis_elems(p), hipMemcpyHostToDevice, p->stream));
     p->have_flags = true;
+    if (int st = flags_changed(p, before)) return st;
     return sync(p);
 }
 

@@ -20,6 +20,8 @@ struct Cand {
 };
 
 // Deconvolution state on the device, read by every kernel of an iteration.
+// The call's parameters live here too, so a recorded graph of iterations
+// serves any lambda, alpha, epsilon and iteration limit.
 struct State {
     int32_t done;       // no further updates (converged or iteration limit)
     int32_t converged;  // the last selected |diff| was below epsilon
@@ -27,6 +29,8 @@ struct State {
     int32_t updated;    // the last selection changed a pixel
     Cand last;          // that change
     float absmax;       // the last selected |diff|
+    float lambda, alpha, epsilon;
+    int32_t max_iter;
     int32_t pad[3];
 };
 
@@ -40,11 +44,10 @@ hipError_t launch_absmax(const Cand* tiles, int ntiles, State* state, hipStream_
 // launch_tiles_all; then launch_iteration repeatedly: the window tiles take
 // the gradient change and are re-evaluated, and one workgroup refreshes the
 // rows the window spans and selects the next update.
-hipError_t launch_first_select(const Cand* tiles, Cand* rows, int H, int W, State* state, float* x, float epsilon,
-                               int max_iter, hipStream_t st);
+// lambda, alpha, epsilon and the iteration limit come from the state.
+hipError_t launch_first_select(const Cand* tiles, Cand* rows, int H, int W, State* state, float* x, hipStream_t st);
 hipError_t launch_iteration(float* x, float* g, const float* amap, const float* psf2, int p2h, int p2w, int H, int W,
-                            float lambda, float alpha, Cand* tiles, Cand* rows, State* state, float epsilon,
-                            int max_iter, hipStream_t st);
+                            Cand* tiles, Cand* rows, State* state, hipStream_t st);
 
 // PaddedConvolver / CalcPaddedFourierCorrelation pieces (complex double
 // buffers of N0 x N1).

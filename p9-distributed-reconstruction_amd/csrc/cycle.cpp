@@ -335,8 +335,17 @@ struct p9cd {
     cd::Cand* d_tiles = nullptr;
     cd::Cand* d_rows = nullptr;
     cd::State* d_state = nullptr;
+    cd::State* h_state = nullptr;  // pinned host copy of the state (hipHostMalloc)
 
     float* d_io = nullptr;  // [2][H][W] staging for host-array calls
+
+    // kBatch iterations recorded as a graph for one (x, gradients) buffer
+    // pair; the parameters are read from the device state, so every call on
+    // the same buffers replays it
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const float* graph_x = nullptr;
+    const float* graph_g = nullptr;
 };
 
 namespace {
@@ -346,47 +355,71 @@ constexpr int kBatch = 128;  // iterations enqueued between host checks of the s
 int cd_abs_max(p9cd* c, const float* d_x, const float* d_g, float lambda, float alpha, float* out) {
     P9_HIP(cd::launch_tiles_all(d_x, d_g, c->d_amap, c->H, c->W, lambda, alpha, c->d_tiles, c->stream));
     P9_HIP(cd::launch_absmax(c->d_tiles, c->ntiles, c->d_state, c->stream));
-    cd::State s;
-    P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    P9_HIP(hipMemcpyAsync(c->h_state, c->d_state, sizeof(cd::State), hipMemcpyDeviceToHost, c->stream));
     P9_HIP(hipStreamSynchronize(c->stream));
-    *out = s.absmax;
+    *out = c->h_state->absmax;
     return P9IDG_OK;
 }
 
-// Iterations are recorded once per call as a graph of kBatch iterations and
-// replayed until the state says done (launches after that return at once).
-int cd_run(p9cd* c, float* d_x, float* d_g, float lambda, float alpha, int32_t max_iter, float eps, int32_t* conv,
-           int32_t* iters) {
-    cd::State s{};
-    s.done = max_iter <= 0;
-    P9_HIP(hipMemcpyAsync(c->d_state, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
-    P9_HIP(cd::launch_tiles_all(d_x, d_g, c->d_amap, c->H, c->W, lambda, alpha, c->d_tiles, c->stream));
-    P9_HIP(cd::launch_first_select(c->d_tiles, c->d_rows, c->H, c->W, c->d_state, d_x, eps, max_iter, c->stream));
-    P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
-    P9_HIP(hipStreamSynchronize(c->stream));
-    if (s.done && !s.updated) {
-        if (conv) *conv = s.converged;
-        if (iters) *iters = s.iters;
-        return P9IDG_OK;
-    }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+void cd_drop_graph(p9cd* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_x = c->graph_g = nullptr;
+}
+
+// Records kBatch iterations on (d_x, d_g) as a graph, unless the recorded
+// one already works on these buffers.  Called with the stream idle.
+int cd_graph(p9cd* c, float* d_x, float* d_g) {
+    if (c->graph_exec && c->graph_x == d_x && c->graph_g == d_g) return P9IDG_OK;
+    cd_drop_graph(c);
     P9_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; k < kBatch && e == hipSuccess; k++)
-        e = cd::launch_iteration(d_x, d_g, c->d_amap, c->d_psf2, c->p2h, c->p2w, c->H, c->W, lambda, alpha, c->d_tiles,
-                                 c->d_rows, c->d_state, eps, max_iter, c->stream);
+        e = cd::launch_iteration(d_x, d_g, c->d_amap, c->d_psf2, c->p2h, c->p2w, c->H, c->W, c->d_tiles, c->d_rows,
+                                 c->d_state, c->stream);
+    hipGraph_t graph = nullptr;
     const hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
     if (e == hipSuccess) e = e_end;
+    hipGraphExec_t exec = nullptr;
     if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    while (e == hipSuccess && (!s.done || s.updated)) {
-        e = hipGraphLaunch(exec, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(e, "deconvolution graph");
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return hip_fail(e, "deconvolution iterations");
+    c->graph = graph;
+    c->graph_exec = exec;
+    c->graph_x = d_x;
+    c->graph_g = d_g;
+    return P9IDG_OK;
+}
+
+// The first selection, then the recorded iterations replayed until the state
+// says done (launches after that return at once).  The state travels through
+// pinned host memory.
+int cd_run(p9cd* c, float* d_x, float* d_g, float lambda, float alpha, int32_t max_iter, float eps, int32_t* conv,
+           int32_t* iters) {
+    cd::State& s = *c->h_state;
+    s = cd::State{};
+    s.done = max_iter <= 0;
+    s.lambda = lambda;
+    s.alpha = alpha;
+    s.epsilon = eps;
+    s.max_iter = max_iter;
+    P9_HIP(hipMemcpyAsync(c->d_state, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+    P9_HIP(cd::launch_tiles_all(d_x, d_g, c->d_amap, c->H, c->W, lambda, alpha, c->d_tiles, c->stream));
+    P9_HIP(cd::launch_first_select(c->d_tiles, c->d_rows, c->H, c->W, c->d_state, d_x, c->stream));
+    P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    P9_HIP(hipStreamSynchronize(c->stream));
+    if (!(s.done && !s.updated)) {
+        if (int st = cd_graph(c, d_x, d_g)) return st;
+        while (!s.done || s.updated) {
+            P9_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+            P9_HIP(hipMemcpyAsync(&s, c->d_state, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+            P9_HIP(hipStreamSynchronize(c->stream));
+        }
+    }
     if (conv) *conv = s.converged;
     if (iters) *iters = s.iters;
     return P9IDG_OK;
@@ -421,6 +454,11 @@ int p9cd_create(int32_t device, int32_t height, int32_t width, const float* psf,
         (st = alloc(&c->d_rows, (size_t)(height + cd::kTile - 1) / cd::kTile)) ||
         (st = alloc(&c->d_io, 2 * (size_t)height * width)))
         return bail(st);
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_state), sizeof(cd::State), hipHostMallocDefault) != hipSuccess) {
+        c->h_state = nullptr;
+        (void)hipGetLastError();
+        return bail(fail(P9IDG_ERR_ALLOC, "pinned host allocation failed"));
+    }
     // psf2 = PSF.CalcPSFSquared(psf), MaxLipschitz = Residuals.GetMax(psf2)
     std::vector<float> psf2((size_t)c->p2h * c->p2w);
     if ((st = p9psf_squared(device, psf, rows, cols, psf2.data()))) return bail(st);
@@ -438,6 +476,8 @@ int p9cd_destroy(p9cd* c) {
     if (!c) return P9IDG_OK;
     Guard g(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    cd_drop_graph(c);
+    if (c->h_state) (void)hipHostFree(c->h_state);
     release(c->d_amap);
     release(c->d_psf2);
     release(c->d_tiles);
@@ -594,6 +634,7 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const char* obs_nam
         const std::string path = std::string(obs_name) + what + std::to_string(cycle) + ".fits";
         return p9fits_write_image(path.c_str(), P9FITS_F32, host.data(), G, G);
     };
+    bool captured = false;
     for (int32_t cycle = 0; cycle < max_major_cycles; cycle++) {
         if (int st = write_image(d_image, "_dirty_serial_majorCycle", cycle)) return st;
         const auto t0 = clock::now();
@@ -641,7 +682,14 @@ extern "C" int p9idg_reconstruct_serial_cd(p9idg_plan* plan, const char* obs_nam
         // finish it before ToUVW reads the image on the plan's stream
         P9_HIP(hipMemcpyAsync(d_image, b.x, GG * sizeof(float), hipMemcpyDeviceToDevice, nullptr));
         P9_HIP(hipStreamSynchronize(nullptr));
-        if (int st = p9idg_exec_cycle(plan, P9IDG_GRID_WSTACK)) return st;
+        // the device step is recorded once as a hipGraph (capture runs it
+        // eagerly first) and replayed in every later cycle
+        if (!captured) {
+            if (int st = p9idg_capture_cycle(plan, P9IDG_GRID_WSTACK)) return st;
+            captured = true;
+        } else {
+            if (int st = p9idg_launch_cycle(plan)) return st;
+        }
         if (int st = p9idg_sync(plan)) return st;
         if (stats) stats[cycle].imaging_ms = ms(t3, clock::now());
     }

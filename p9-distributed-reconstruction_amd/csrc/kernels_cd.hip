@@ -199,7 +199,7 @@ __device__ Cand wave_best(const Cand* v, int n, int lane) {
 // Rows [ty0, ty1] of the row table from the tile table, then the overall best
 // (4 waves), and the selection of FastSerialCD.cs:84-97 by thread 0.
 __device__ void rows_and_select(const Cand* tiles, Cand* rows, int ntx, int nty, int ty0, int ty1, State* st,
-                                float* x, float epsilon, int max_iter) {
+                                float* x) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int ty = ty0 + wave; ty <= ty1; ty += kThreads / 64) {
         const Cand b = wave_best(tiles + (size_t)ty * ntx, ntx, lane);
@@ -210,6 +210,7 @@ __device__ void rows_and_select(const Cand* tiles, Cand* rows, int ntx, int nty,
     if (wave != 0) return;
     const Cand best = wave_best(rows, nty, lane);
     if (lane != 0) return;
+    const float epsilon = st->epsilon;
     st->absmax = best.diff;
     if (best.diff < epsilon || best.idx == INT_MAX) {
         st->converged = best.diff < epsilon;
@@ -221,14 +222,14 @@ __device__ void rows_and_select(const Cand* tiles, Cand* rows, int ntx, int nty,
     st->last = best;
     st->updated = 1;
     st->iters += 1;
-    if (st->iters >= max_iter) st->done = 1;
+    if (st->iters >= st->max_iter) st->done = 1;
 }
 
 // All rows and the first selection (after tiles_all_kernel).
 __global__ __launch_bounds__(kThreads) void first_select_kernel(const Cand* tiles, Cand* rows, int ntx, int nty,
-                                                                State* st, float* x, float epsilon, int max_iter) {
+                                                                State* st, float* x) {
     if (st->done) return;
-    rows_and_select(tiles, rows, ntx, nty, 0, nty - 1, st, x, epsilon, max_iter);
+    rows_and_select(tiles, rows, ntx, nty, 0, nty - 1, st, x);
 }
 
 // Window half of an iteration: each block owns one tile of the PSF^2 window
@@ -237,10 +238,11 @@ __global__ __launch_bounds__(kThreads) void first_select_kernel(const Cand* tile
 // last selection is applied even when that selection reached the iteration
 // limit (the reference updates before it stops).
 __global__ __launch_bounds__(kThreads) void window_kernel(float* x, float* g, const float* amap, const float* psf2,
-                                                          int p2h, int p2w, int H, int W, float lambda, float alpha,
-                                                          Cand* tiles, const State* st) {
+                                                          int p2h, int p2w, int H, int W, Cand* tiles,
+                                                          const State* st) {
     __shared__ Cand sh[kThreads / 64];
     if (!st->updated) return;  // uniform
+    const float lambda = st->lambda, alpha = st->alpha;
     const int ntx = (W + kTile - 1) / kTile;
     const int y0 = st->last.idx / W, x0 = st->last.idx % W;
     const float d = __fsub_rn(st->last.xold, st->last.xnew);
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void window_kernel(float* x, float* g, co
 // Selection half: refresh the rows the window spans and select the next
 // update (one workgroup).
 __global__ __launch_bounds__(kThreads) void row_select_kernel(const Cand* tiles, Cand* rows, int p2h, int H, int W,
-                                                              State* st, float* x, float epsilon, int max_iter) {
+                                                              State* st, float* x) {
     if (!st->updated) return;  // uniform
     if (st->done) {
         __syncthreads();
@@ -277,8 +279,7 @@ __global__ __launch_bounds__(kThreads) void row_select_kernel(const Cand* tiles,
     const int y0 = st->last.idx / W;
     const int r0 = max(y0 - p2h / 2, 0), r1 = min(y0 - p2h / 2 + p2h, H);
     __syncthreads();  // every thread has read the state before thread 0 rewrites it
-    rows_and_select(tiles, rows, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile, r0 / kTile, (r1 - 1) / kTile, st, x,
-                    epsilon, max_iter);
+    rows_and_select(tiles, rows, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile, r0 / kTile, (r1 - 1) / kTile, st, x);
 }
 
 __global__ void zero_u32_kernel(unsigned* p) { *p = 0u; }
@@ -343,21 +344,17 @@ hipError_t launch_extract(const double2* buf, int N1, int yh, int xh, double nor
     return hipGetLastError();
 }
 
-hipError_t launch_first_select(const Cand* tiles, Cand* rows, int H, int W, State* state, float* x, float epsilon,
-                               int max_iter, hipStream_t st) {
+hipError_t launch_first_select(const Cand* tiles, Cand* rows, int H, int W, State* state, float* x, hipStream_t st) {
     hipLaunchKernelGGL(first_select_kernel, dim3(1), dim3(kThreads), 0, st, tiles, rows, (W + kTile - 1) / kTile,
-                       (H + kTile - 1) / kTile, state, x, epsilon, max_iter);
+                       (H + kTile - 1) / kTile, state, x);
     return hipGetLastError();
 }
 
 hipError_t launch_iteration(float* x, float* g, const float* amap, const float* psf2, int p2h, int p2w, int H, int W,
-                            float lambda, float alpha, Cand* tiles, Cand* rows, State* state, float epsilon,
-                            int max_iter, hipStream_t st) {
+                            Cand* tiles, Cand* rows, State* state, hipStream_t st) {
     const dim3 grid((p2w + kTile - 1) / kTile + 1, (p2h + kTile - 1) / kTile + 1);
-    hipLaunchKernelGGL(window_kernel, grid, dim3(kThreads), 0, st, x, g, amap, psf2, p2h, p2w, H, W, lambda, alpha,
-                       tiles, state);
-    hipLaunchKernelGGL(row_select_kernel, dim3(1), dim3(kThreads), 0, st, tiles, rows, p2h, H, W, state, x, epsilon,
-                       max_iter);
+    hipLaunchKernelGGL(window_kernel, grid, dim3(kThreads), 0, st, x, g, amap, psf2, p2h, p2w, H, W, tiles, state);
+    hipLaunchKernelGGL(row_select_kernel, dim3(1), dim3(kThreads), 0, st, tiles, rows, p2h, H, W, state, x);
     return hipGetLastError();
 }
 

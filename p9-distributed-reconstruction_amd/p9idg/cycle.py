@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, ptr
-from .idg import _vis_arg
+from .idg import _flags_arg, _vis_arg
 
 __all__ = ["PSF", "PaddedConvolver", "FastSerialCD", "DeconvolutionResult", "MajorCycle", "CYCLE_STATS_DTYPE"]
 
@@ -206,8 +206,8 @@ class MajorCycle:
         structured array, one row per cycle run).  With obsName, every cycle
         writes <obsName>_dirty_serial_majorCycle<k>.fits and
         <obsName>_model_serial_majorCycle<k>.fits (MajorCycle.cs:91,117)."""
-        vis, vtype = _vis_arg(visibilities)
-        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        vis, vtype = _vis_arg(visibilities, plan.shape)
+        fl = _flags_arg(flags, plan.shape)
         G = plan.c.GridSize
         x = np.zeros((G, G), dtype=np.float32)
         stats = np.zeros(max(int(maxMajorCycle), 1), dtype=CYCLE_STATS_DTYPE)

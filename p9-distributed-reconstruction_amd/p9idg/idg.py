@@ -144,11 +144,26 @@ def _as_partition(metadata):
     raise TypeError("metadata must be a Partition or a SUBGRID_DTYPE array")
 
 
-def _vis_arg(vis):
+def _vis_arg(vis, shape=None):
     vis = np.asarray(vis)
+    _expect(vis, shape, "visibilities")
     if vis.dtype == np.complex64:
         return np.ascontiguousarray(vis), _lib.C64
     return np.ascontiguousarray(vis, dtype=np.complex128), _lib.C128
+
+
+def _expect(a, shape, name):
+    """The C ABI reads exactly prod(shape) elements of each input: anything
+    else is refused here, as the reference's shaped C# arrays would throw."""
+    if shape is not None and tuple(np.shape(a)) != tuple(shape):
+        raise ValueError("%s must have shape %s, got %s" % (name, tuple(shape), tuple(np.shape(a))))
+
+
+def _flags_arg(flags, shape):
+    if flags is None:
+        return None
+    _expect(flags, shape, "flags")
+    return np.ascontiguousarray(flags, dtype=np.uint8)
 
 
 class Plan:
@@ -208,67 +223,78 @@ class Plan:
     def set_observation(self, uvw, frequencies, metadata):
         uvw = np.ascontiguousarray(uvw, dtype=np.float64)
         freq = np.ascontiguousarray(frequencies, dtype=np.float64)
+        if uvw.ndim != 3 or uvw.shape[2] != 3:
+            raise ValueError("uvw must have shape (baselines, timesteps, 3), got %s" % (uvw.shape,))
+        if freq.ndim != 1:
+            raise ValueError("frequencies must be one-dimensional, got shape %s" % (freq.shape,))
         meta = np.ascontiguousarray(_as_partition(metadata))
         B, T = uvw.shape[0], uvw.shape[1]
         check(_lib.idg().p9idg_set_observation(self._h, ptr(uvw), B, T, ptr(freq), len(freq), ptr(meta), len(meta)))
         self.shape = (B, T, len(freq))
 
     # -- host-buffer drop-ins (IDG.cs) --
+    @property
+    def _grid_shape(self):
+        return (self.c.GridSize, self.c.GridSize)
+
     def grid(self, vis):
-        v, t = _vis_arg(vis)
+        v, t = _vis_arg(vis, self.shape)
         G = self.c.GridSize
         out = np.zeros((G, G), dtype=np.complex128)
         check(_lib.idg().p9idg_grid(self._h, ptr(v), t, ptr(out)))
         return out
 
     def gridw(self, vis):
-        v, t = _vis_arg(vis)
+        v, t = _vis_arg(vis, self.shape)
         G, L = self.c.GridSize, self.c.WLayerCount
         out = np.zeros((L, G, G), dtype=np.complex128)
         check(_lib.idg().p9idg_gridw(self._h, ptr(v), t, ptr(out)))
         return out
 
     def degrid(self, grid):
+        _expect(grid, self._grid_shape, "grid")
         g = np.ascontiguousarray(grid, dtype=np.complex128)
         out = np.zeros(self.shape, dtype=np.complex128)
         check(_lib.idg().p9idg_degrid(self._h, ptr(g), ptr(out)))
         return out
 
     def degridw(self, grid):
+        _expect(grid, self._grid_shape, "grid")
         g = np.ascontiguousarray(grid, dtype=np.complex128)
         out = np.zeros(self.shape, dtype=np.complex128)
         check(_lib.idg().p9idg_degridw(self._h, ptr(g), ptr(out)))
         return out
 
     def grid_psf(self, flags=None):
-        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        fl = _flags_arg(flags, self.shape)
         G = self.c.GridSize
         out = np.zeros((G, G), dtype=np.complex128)
         check(_lib.idg().p9idg_grid_psf(self._h, ptr(fl), ptr(out)))
         return out
 
     def to_image(self, vis):
-        v, t = _vis_arg(vis)
+        v, t = _vis_arg(vis, self.shape)
         G = self.c.GridSize
         out = np.zeros((G, G), dtype=np.float64)
         check(_lib.idg().p9idg_to_image(self._h, ptr(v), t, ptr(out)))
         return out
 
     def calculate_psf(self, flags=None):
-        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        fl = _flags_arg(flags, self.shape)
         G = self.c.GridSize
         out = np.zeros((G, G), dtype=np.float64)
         check(_lib.idg().p9idg_calculate_psf(self._h, ptr(fl), ptr(out)))
         return out
 
     def gridw_to_image(self, vis):
-        v, t = _vis_arg(vis)
+        v, t = _vis_arg(vis, self.shape)
         G = self.c.GridSize
         out = np.zeros((G, G), dtype=np.float32)
         check(_lib.idg().p9idg_gridw_to_image(self._h, ptr(v), t, ptr(out)))
         return out
 
     def image_to_vis(self, image):
+        _expect(image, self._grid_shape, "image")
         im = np.ascontiguousarray(image, dtype=np.float32)
         out = np.zeros(self.shape, dtype=np.complex128)
         check(_lib.idg().p9idg_image_to_vis(self._h, ptr(im), ptr(out)))
@@ -276,11 +302,11 @@ class Plan:
 
     # -- device-resident stages --
     def upload_vis(self, vis):
-        v, t = _vis_arg(vis)
+        v, t = _vis_arg(vis, self.shape)
         check(_lib.idg().p9idg_upload_vis(self._h, ptr(v), t))
 
     def set_psf_vis(self, flags=None):
-        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        fl = _flags_arg(flags, self.shape)
         check(_lib.idg().p9idg_set_psf_vis(self._h, ptr(fl)))
 
     def exec_grid(self, mode=_lib.GRID_WSTACK):
@@ -299,7 +325,7 @@ class Plan:
         check(_lib.idg().p9idg_sync(self._h))
 
     def set_flags(self, flags=None):
-        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        fl = _flags_arg(flags, self.shape)
         check(_lib.idg().p9idg_set_flags(self._h, ptr(fl)))
 
     def exec_residual(self):
@@ -330,6 +356,7 @@ class Plan:
         return out
 
     def upload_model_grid(self, grid):
+        _expect(grid, self._grid_shape, "model grid")
         g = np.ascontiguousarray(grid, dtype=np.complex64)
         check(_lib.idg().p9idg_upload_model_grid(self._h, ptr(g)))
 
@@ -345,6 +372,7 @@ class Plan:
         return out
 
     def upload_image(self, image):
+        _expect(image, self._grid_shape, "image")
         im = np.ascontiguousarray(image, dtype=np.float32)
         check(_lib.idg().p9idg_upload_image(self._h, ptr(im)))
 
@@ -488,6 +516,11 @@ class IDG:
         return _plan(c, metadata, uvw, frequencies).calculate_psf(flags)
 
 
+def _square(a, name):
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError("%s must be square (G, G), got shape %s" % (name, a.shape))
+
+
 def _fft_plan(G, L=1):
     return Plan(GriddingConstants(1, G, 8 if G > 8 else 2, 0, 1, 1e-5, L, 0.0))
 
@@ -498,6 +531,8 @@ class FFT:
     @staticmethod
     def WStackIFFTFloat(grid, visibilityCount):
         layers = np.ascontiguousarray(np.stack(list(grid)) if isinstance(grid, (list, tuple)) else grid, dtype=np.complex128)
+        if layers.ndim != 3 or layers.shape[1] != layers.shape[2]:
+            raise ValueError("w-layers must have shape (L, G, G), got %s" % (layers.shape,))
         L, G = layers.shape[0], layers.shape[1]
         with _fft_plan(G, L) as p:
             out = np.zeros((G, G), dtype=np.float32)
@@ -507,6 +542,7 @@ class FFT:
     @staticmethod
     def Forward(image, norm=1.0):
         im = np.ascontiguousarray(image, dtype=np.float32)
+        _square(im, "image")
         G = im.shape[0]
         with _fft_plan(G) as p:
             out = np.zeros((G, G), dtype=np.complex128)
@@ -516,6 +552,7 @@ class FFT:
     @staticmethod
     def Backward(grid, visibilityCount):
         g = np.ascontiguousarray(grid, dtype=np.complex128)
+        _square(g, "grid")
         G = g.shape[0]
         with _fft_plan(G) as p:
             out = np.zeros((G, G), dtype=np.float64)
@@ -531,6 +568,7 @@ class FFT:
             return a
         if not a.flags["C_CONTIGUOUS"]:
             raise ValueError("Shift needs a C-contiguous array")
+        _square(a, "array")
         n = a.shape[0]
         if a.dtype == np.float32:
             check(_lib.idg().p9idg_shift_f32(ptr(a), n))
@@ -549,7 +587,8 @@ class Visibilities:
         """Common.cs:305-318: vis0 - vis1, 0 where flagged."""
         v0 = np.ascontiguousarray(vis0, dtype=np.complex128)
         v1 = np.ascontiguousarray(vis1, dtype=np.complex128)
-        fl = None if flag is None else np.ascontiguousarray(flag, dtype=np.uint8)
+        _expect(v1, v0.shape, "vis1")
+        fl = _flags_arg(flag, v0.shape)
         out = np.zeros_like(v0)
         check(_lib.idg().p9idg_substract(ptr(v0), ptr(v1), ptr(fl), v0.size, ptr(out)))
         return out

@@ -82,6 +82,32 @@ def test_fast_serial_cd_matches_oracle_bit_for_bit(O, H, ph, lam, alpha, iters, 
     assert np.array_equal(g, gr)
 
 
+def test_fast_serial_cd_reuses_its_graph_across_parameters(O):
+    """One deconvolver, three calls on the same buffers with different
+    lambda, alpha, epsilon and iteration limits: the recorded iterations are
+    replayed with each call's parameters (read from the device state), and
+    every call equals the oracle bit for bit, continuing from the last."""
+    H, ph = 96, 16
+    rng = np.random.default_rng(31)
+    psf = beam(ph, ph / 6.0, rng)
+    sky = np.zeros((H, H), np.float32)
+    for _ in range(10):
+        sky[rng.integers(4, H - 4), rng.integers(4, H - 4)] = rng.uniform(0.2, 1.0)
+    with p9idg.PaddedConvolver(psf, (H, H)) as conv:
+        g0 = conv.Convolve(conv.Convolve(sky))
+    psf2 = p9idg.PSF.CalcPSFSquared(psf)
+    amap = p9idg.PSF.CalcAMap(psf, (H, H))
+    x = np.zeros((H, H), np.float32)
+    g = g0.copy()
+    xr, gr = x.copy(), g.copy()
+    with p9idg.FastSerialCD((H, H), psf) as cd:
+        for lam, alpha, iters, eps in [(0.05, 0.01, 300, 1e-5), (0.5, 0.2, 90, 1e-6), (0.02, 0.05, 700, 1e-4)]:
+            res = cd.Deconvolve(x, g, lam, alpha, iters, eps)
+            xr, gr, conv_r, it_r = O.cd_deconvolve(amap, psf2, xr, gr, lam, alpha, iters, eps)
+            assert (res.IterationCount, res.Converged) == (it_r, conv_r)
+            assert np.array_equal(x, xr) and np.array_equal(g, gr)
+
+
 @pytest.mark.parametrize("kernel", ["fp32", "mfma_bf16"])
 def test_reconstruct_serial_cd_against_oracle(O, kernel, tmp_path):
     cell = obs.ARCSEC * 20.0

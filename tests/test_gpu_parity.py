@@ -314,6 +314,76 @@ def test_deterministic_adder_plain_grid_and_graph(small, small_ref, O):
         assert e.value.status == p9idg._lib.ERR_STATE
 
 
+def test_deterministic_graph_survives_other_grid_modes():
+    """A captured deterministic cycle keeps its own tile lists: gridding in
+    the other modes between replays builds their lists beside it, and every
+    replay still equals the eager step bit for bit (4 w-layers, negative w)."""
+    ob = obs.custom(G=256, S=32, K=8, L=4, n_ant=10, T=40, C=6, seed=123, core_diam=1500.0)
+    part = ob.partition()
+    m = part.subgrids
+    assert (m["WLambda"] < 0).any() and (m["WLambda"] > 0).any()
+    G, S = ob.c.GridSize, ob.c.SubgridSize
+    plain_ok = bool(((m["UPixel"] >= 0) & (m["UPixel"] < G - S) & (m["VPixel"] >= 0) & (m["VPixel"] < G - S)).all())
+    others = [p9idg.GRID_WSUM, p9idg.GRID_PLAIN if plain_ok else p9idg.GRID_WSUM, p9idg.GRID_WSUM]
+    model = np.zeros((G, G), np.float32)
+    model[G // 2 + 7, G // 2 - 30] = 0.4
+    with p9idg.Plan(ob.c, kernel="mfma_bf16") as plan:
+        plan.set_deterministic(True)
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.upload_image(model)
+        plan.exec_cycle(p9idg.GRID_WSTACK)
+        eager = plan.download_image()
+        plan.upload_image(model)
+        plan.capture_cycle(p9idg.GRID_WSTACK)
+        assert np.array_equal(plan.download_image(), eager)
+        for rep, other in enumerate(others):
+            plan.exec_grid(other)  # prepares the other mode's tile lists
+            plan.sync()
+            plan.upload_image(model)
+            plan.launch_cycle()
+            plan.sync()
+            assert np.array_equal(plan.download_image(), eager), (rep, other)
+
+
+def test_flags_after_capture_drop_the_graph(small, O):
+    """The captured cycle holds the flag buffer it was recorded with: setting
+    flags for the first time or clearing them drops the graph (launch is then
+    refused); new flag contents in the same buffer are read by the replay."""
+    ob, part, oc = small
+    G = ob.c.GridSize
+    rng = np.random.default_rng(8)
+    model = np.zeros((G, G), np.float32)
+    model[G // 2 - 12, G // 2 + 20] = 0.7
+    fl1 = rng.random(ob.vis.shape) < 0.1
+    fl2 = rng.random(ob.vis.shape) < 0.1
+
+    def ref(flags):
+        mv = O.image_to_vis(oc, part.subgrids, model, ob.uvw, ob.freq)
+        return O.gridw_to_image(oc, part.subgrids, O.substract(ob.vis, mv, flags), ob.uvw, ob.freq)
+
+    with p9idg.Plan(ob.c, kernel="mfma_bf16") as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.upload_image(model)
+        plan.capture_cycle(p9idg.GRID_WSTACK)
+        plan.set_flags(fl1)  # first flags: the graph recorded none
+        with pytest.raises(p9idg.P9IDGError) as e:
+            plan.launch_cycle()
+        assert e.value.status == p9idg._lib.ERR_STATE
+        plan.upload_image(model)
+        plan.capture_cycle(p9idg.GRID_WSTACK)
+        plan.set_flags(fl2)  # same buffer, new contents: the graph stays
+        plan.upload_image(model)
+        plan.launch_cycle()
+        plan.sync()
+        assert rel_err(plan.download_image(), ref(fl2)) <= TOL
+        plan.set_flags(None)  # cleared: dropped again
+        with pytest.raises(p9idg.P9IDGError) as e:
+            plan.launch_cycle()
+        assert e.value.status == p9idg._lib.ERR_STATE
+
+
 def test_bad_metadata_rejected():
     ob = obs.custom(G=128, S=16, K=4, L=2, n_ant=6, T=10, C=3, seed=6)
     part = ob.partition()
