@@ -3,15 +3,24 @@
 
 Workload: the "large" observation (3072^2 image, 32 w-layers, 64 antennas =
 2016 baselines x 2000 timesteps x 8 channels, subgrid 32, kernel 8), inputs
-resident in HBM.  One step = IDG.GridW (zero the 32 layers, fused gridder ->
-subgrid FFT -> w-stack adder) + IDG.DeGridW (fused splitter -> subgrid FFT ->
-degridder) over every gridded visibility.  Visibilities per second count the
-visibilities the partition places on subgrids; the total is reported beside.
+resident in HBM.  One step = IDG.GridW (zero the layers' footprint, fused
+gridder -> subgrid FFT -> w-stack adder) + IDG.DeGridW (fused splitter ->
+subgrid FFT -> degridder) over every gridded visibility.  Visibilities per
+second count the visibilities the partition places on subgrids; the total is
+reported beside.
 
-Prints one JSON line (see the keys below).  The CPU baseline is the FP64
-oracle (a C++ restatement of the reference's Parallel.For path) timed on the
-host cores on a stride sample of the subgrids; the same sample gives the
-error of the GPU result against the oracle.
+Prints one JSON line.  Beside the headline it carries:
+  vs_fp32         the same step with the two FP32 vector-ALU kernels
+                  (per-pixel phasors, and the separable phasors of the MFMA
+                  form), each autotuned, timed in this run
+  hbm             achieved GB/s of the HBM-bound stages (target zeroing,
+                  ToImage, ToUVW), HIP events over repeated runs
+  error_vs_oracle the timed plan (its autotuned launch shape) against the FP64
+                  oracle on the whole workload: dirty image, PSF, degridded
+                  visibilities (max and RMS relative error)
+  cpu_baseline    the FP64 oracle (a C++ restatement of the reference's
+                  Parallel.For path) timed on the host: grid + degrid of the
+                  whole workload on the host threads, and a 1-thread sample
 """
 import argparse
 import hashlib
@@ -26,8 +35,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "p9-distributed-reconstruction_amd"))
 
 METRIC = "Mvis/s grid+degrid, 3072² 32 w-layers: split-BF16 MFMA vs FP32, ≤1e-5 err"
-PEAK_TFLOPS = {"fp32": 157.3, "mfma_bf16": 2500.0, "mfma_fp16": 2500.0}
-DTYPE = {"fp32": "f32", "mfma_bf16": "bf16 hi/lo split, f32 accumulate", "mfma_fp16": "f16 hi/lo split, f32 accumulate"}
+PEAK_TFLOPS = {"fp32": 157.3, "fp32_sep": 157.3, "mfma_bf16": 2500.0, "mfma_fp16": 2500.0}
+BOUND = {"fp32": "valu", "fp32_sep": "valu", "mfma_bf16": "mfma", "mfma_fp16": "mfma"}
+DTYPE = {"fp32": "f32", "fp32_sep": "f32", "mfma_bf16": "bf16 hi/lo split, f32 accumulate",
+         "mfma_fp16": "f16 hi/lo split, f32 accumulate"}
+HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
@@ -41,6 +53,24 @@ def host_threads():
     except AttributeError:
         pass
     return max(1, min(n, 16))
+
+
+def cpu_info():
+    """nproc, the CPUs this process may use, and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "usable_cpus": usable, "cpu_model": model}
 
 
 def model_image(ob):
@@ -63,131 +93,152 @@ def sample_subgrids(meta, C, target_vis):
     return np.ascontiguousarray(meta[::k])
 
 
-def cpu_baseline_and_error(args, ob, part, model_grid_c64, kernel):
-    """Times the FP64 oracle on a bounded sample and compares the GPU on it."""
+def rel_errs(got, ref, mask=None):
+    """max |got - ref| / max |ref| and the RMS ratio (over mask)."""
+    d = np.abs(got.astype(np.complex128) - ref)
+    r = np.abs(ref)
+    if mask is not None:
+        d, r = d[mask], r[mask]
+    return {"max_rel": float(d.max() / r.max()), "rms_rel": float(np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(r ** 2)))}
+
+
+def covered_mask(meta, shape):
+    m = np.zeros(shape[:2], dtype=bool)
+    for s in meta:
+        m[s["baselineIdx"], s["timeSampleStart"]:s["timeSampleStart"] + s["timeSampleCount"]] = True
+    return m
+
+
+def oracle_constants(O, c):
+    return O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize, c.MaxTimestepsPerSubgrid,
+                            float(c.ScaleArcSec), c.WLayerCount, c.WStep)
+
+
+def oracle_checks(ob, part, gpu, model_grid_c64):
+    """The FP64 oracle on the whole workload: the references of the GPU's dirty
+    image, PSF and degridded visibilities, and its grid + degrid time on the
+    host threads (the CPU baseline), plus a 1-thread rate on a stride sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    import p9idg
 
     threads = host_threads()
     O.set_threads(threads)
     c = ob.c
-    oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize,
-                          c.MaxTimestepsPerSubgrid, float(c.ScaleArcSec), c.WLayerCount, c.WStep)
-    C = len(ob.freq)
+    oc = oracle_constants(O, c)
     meta = part.subgrids
+    C = len(ob.freq)
     vis128 = ob.vis.astype(np.complex128)
     model128 = model_grid_c64.astype(np.complex128)
-    g = np.array(model128, copy=True)
-    O.shift(g)  # IDG.DeGridW shifts the grid first (IDG.cs:44)
-
-    def run(smp):
-        t0 = time.perf_counter()
-        sub = O.gridder_forward(oc, smp, ob.uvw, vis128, ob.freq)
-        sub = O.subgrid_fft_forward(oc, sub)
-        lay = O.adderw_add(oc, smp, sub)
-        t1 = time.perf_counter()
-        sub = O.adderw_split(oc, smp, g)
-        sub = O.subgrid_fft_backward(oc, sub)
-        v = O.gridder_backward(oc, smp, sub, ob.uvw, ob.freq)
-        t2 = time.perf_counter()
-        return lay, v, t1 - t0, t2 - t1
-
-    # calibrate the oracle's rate on about 1/64 of the workload (whole
-    # pipeline, both directions), then size the timed sample to the target
-    probe = sample_subgrids(meta, C, int(meta["timeSampleCount"].sum()) * C // 64)
-    _, _, pa, pb = run(probe)
-    del _
-    rate = int(probe["timeSampleCount"].sum()) * C / max(pa + pb, 1e-6)
-    sample = sample_subgrids(meta, C, max(20000, int(rate * args.cpu_seconds)))
-    n_vis = int(sample["timeSampleCount"].sum()) * C
-    log("cpu baseline: %d subgrids, %d visibilities, %d threads" % (len(sample), n_vis, threads))
-    layers_ref, vis_ref, tg, td = run(sample)
-    del g
-    for k in range(layers_ref.shape[0]):
-        O.shift(layers_ref[k])  # IDG.GridW shifts each layer (IDG.cs:37)
+    n_vis = int(meta["timeSampleCount"].sum()) * C
+    t0 = time.perf_counter()
+    layers = O.gridw(oc, meta, vis128, ob.uvw, ob.freq)  # IDG.GridW (IDG.cs:32-40)
+    t_grid = time.perf_counter() - t0
+    dirty = O.wstack_ifft_float(layers, c.VisibilitiesCount)  # FFT.WStackIFFTFloat + Shift (MajorCycle.cs:88-90)
+    O.shift(dirty)
+    del layers
+    log("oracle: dirty image (grid %.1f s)" % t_grid)
+    psf = O.gridw_to_image(oc, meta, np.ones_like(vis128), ob.uvw, ob.freq)
+    log("oracle: psf")
+    t0 = time.perf_counter()
+    vis = O.degridw(oc, meta, model128, ob.uvw, ob.freq)  # IDG.DeGridW (IDG.cs:42-50)
+    t_degrid = time.perf_counter() - t0
+    log("oracle: degridded visibilities (%.1f s)" % t_degrid)
+    mask = covered_mask(meta, ob.vis.shape)
+    err = {
+        "dirty_image": rel_errs(gpu["dirty"], dirty),
+        "psf": rel_errs(gpu["psf"], psf),
+        "degridded_vis": rel_errs(gpu["vis"], vis, mask),
+        "uncovered_vis_all_zero": bool(np.abs(gpu["vis"][~mask]).max() == 0.0),
+    }
+    del dirty, psf, vis
+    # the same grid + degrid on one thread, on a stride sample of the subgrids
+    smp = sample_subgrids(meta, C, n_vis // 200)
+    n_smp = int(smp["timeSampleCount"].sum()) * C
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    lay1 = O.gridw(oc, smp, vis128, ob.uvw, ob.freq)
+    O.degridw(oc, smp, model128, ob.uvw, ob.freq)
+    t_one = time.perf_counter() - t0
+    del lay1
+    O.set_threads(threads)
+    info = cpu_info()
     cpu = {
-        "value": n_vis / (tg + td) / 1e6,
+        "value": n_vis / (t_grid + t_degrid) / 1e6,
         "unit": "Mvis/s",
         "cores": threads,
         "kind": "port",
-        "sample": "every %d-th subgrid of the workload: %d subgrids, %d visibilities; FP64 grid %.1f s + degrid %.1f s"
-        % (max(1, len(meta) // max(len(sample), 1)), len(sample), n_vis, tg, td),
-    }
-
-    # GPU on the same sample
-    with p9idg.Plan(c, kernel=kernel) as plan:
-        plan.set_observation(ob.uvw, ob.freq, sample)
-        plan.upload_vis(ob.vis)
-        plan.exec_grid(p9idg.GRID_WSTACK)
-        layers = plan.download_grid(p9idg.GRID_WSTACK)
-        plan.upload_model_grid(model_grid_c64)
-        plan.exec_degrid(p9idg.DEGRID_WSTACK)
-        vis = plan.download_model_vis()
-    err_grid = float(np.abs(layers - layers_ref).max() / np.abs(layers_ref).max())
-    covered = np.zeros(vis.shape[:2], dtype=bool)
-    for s in sample:
-        covered[s["baselineIdx"], s["timeSampleStart"]:s["timeSampleStart"] + s["timeSampleCount"]] = True
-    dv = np.abs(vis - vis_ref)[covered]
-    err_vis = float(dv.max() / np.abs(vis_ref[covered]).max())
-    rms_vis = float(np.sqrt(np.mean(dv ** 2)) / np.sqrt(np.mean(np.abs(vis_ref[covered]) ** 2)))
-    err = {
-        "sample_subgrids": int(len(sample)),
-        "gridw_layers_max_rel": err_grid,
-        "degridw_vis_max_rel": err_vis,
-        "degridw_vis_rms_rel": rms_vis,
+        "sample": "the whole workload (%d gridded visibilities): FP64 GridW %.1f s + DeGridW %.1f s on %d host threads"
+                  % (n_vis, t_grid, t_degrid, threads),
+        "single_thread": {"value": n_smp / t_one / 1e6, "unit": "Mvis/s",
+                          "sample": "every %d-th subgrid: %d visibilities, grid + degrid %.1f s"
+                                    % (max(1, len(meta) // max(len(smp), 1)), n_smp, t_one)},
+        "nproc": info["nproc"],
+        "usable_cpus": info["usable_cpus"],
+        "cpu_model": info["cpu_model"],
     }
     return cpu, err
 
 
-def full_accuracy(ob, part, model_grid_c64, kernel):
-    """Max and RMS relative error of the dirty image, the PSF and the degridded
-    visibilities of the whole workload against the FP64 oracle (minutes of CPU)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def time_steps(plan, steps, warmup):
+    """W untimed and K timed grid + degrid steps; wall time per step and the
+    HIP-event averages of the gridder, degridder and zero-fill spans."""
     import p9idg
 
-    O.set_threads(host_threads())
-    c = ob.c
-    oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize,
-                          c.MaxTimestepsPerSubgrid, float(c.ScaleArcSec), c.WLayerCount, c.WStep)
-    meta = part.subgrids
-    vis128 = ob.vis.astype(np.complex128)
-
-    def errs(a, b, mask=None):
-        d = np.abs(a.astype(np.float64) - b) if mask is None else np.abs(a - b)[mask]
-        r = np.abs(b) if mask is None else np.abs(b)[mask]
-        return {"max_rel": float(d.max() / r.max()), "rms_rel": float(np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(r ** 2)))}
-
-    out = {}
-    with p9idg.Plan(c, kernel=kernel) as plan:
-        plan.set_observation(ob.uvw, ob.freq, part)
-        plan.upload_vis(ob.vis)
+    def step():
         plan.exec_grid(p9idg.GRID_WSTACK)
-        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
-        dirty = plan.download_image()
-        plan.set_psf_vis(None)
-        plan.exec_grid(p9idg.GRID_WSTACK)
-        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
-        psf = plan.download_image()
-        plan.upload_model_grid(model_grid_c64)
         plan.exec_degrid(p9idg.DEGRID_WSTACK)
-        vis = plan.download_model_vis()
-    t = time.perf_counter()
-    ref = O.gridw_to_image(oc, meta, vis128, ob.uvw, ob.freq)
-    out["dirty_image"] = errs(dirty, ref)
-    log("full accuracy: dirty image %.1f s" % (time.perf_counter() - t))
-    ref = O.gridw_to_image(oc, meta, np.ones_like(vis128), ob.uvw, ob.freq)
-    out["psf"] = errs(psf, ref)
-    log("full accuracy: psf done")
-    g = model_grid_c64.astype(np.complex128)
-    ref_v = O.degridw(oc, meta, g, ob.uvw, ob.freq)
-    covered = np.zeros(vis.shape[:2], dtype=bool)
-    for s in meta:
-        covered[s["baselineIdx"], s["timeSampleStart"]:s["timeSampleStart"] + s["timeSampleCount"]] = True
-    out["degridded_vis"] = errs(vis.astype(np.complex128), ref_v, covered)
-    log("full accuracy: degrid done")
-    return out
+
+    for _ in range(warmup):
+        step()
+    plan.sync()
+    plan.timing_reset(True)
+    plan.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    plan.sync()
+    t1 = time.perf_counter()
+    g_ms, g_n, d_ms, d_n = plan.timing_totals()
+    z_ms, _ = plan.timing_stage(p9idg.STAGE_ZERO)
+    plan.timing_reset(False)
+    return {"ms_per_step": (t1 - t0) * 1e3 / steps, "gridder_ms": g_ms / max(g_n, 1),
+            "degridder_ms": d_ms / max(d_n, 1), "zero_ms": z_ms / steps}
+
+
+def wstack_footprint(meta, G, S, L):
+    """Per-layer bounding boxes of the w-stacked adder (AdderWStack.cs:29-70):
+    summed area and the union box's area, in pixels."""
+    X = meta["UPixel"].astype(np.int64)
+    Y = meta["VPixel"].astype(np.int64)
+    W = meta["WLambda"].astype(np.int64)
+    neg = W < 0
+    X = np.where(neg, G - X - S + 1, X)
+    Y = np.where(neg, G - Y - S + 1, Y)
+    W = np.where(neg, -W - 1, W)
+    ok = (X >= 1) & (X < G - S) & (Y >= 1) & (Y < G - S) & (W >= 0) & (W < L)
+    area = 0
+    for l in range(L):
+        k = ok & (W == l)
+        if k.any():
+            area += int((Y[k].max() + S - Y[k].min()) * (X[k].max() + S - X[k].min()))
+    union = int((Y[ok].max() + S - Y[ok].min()) * (X[ok].max() + S - X[ok].min())) if ok.any() else 0
+    return area, union
+
+
+def comparator(ob, part, kernel, steps, warmup, tune):
+    """The same step with another kernel on its own (autotuned) plan."""
+    import p9idg
+
+    with p9idg.Plan(ob.c, kernel=kernel) as p:
+        p.set_observation(ob.uvw, ob.freq, part)
+        p.upload_vis(ob.vis)
+        p.upload_image(model_image(ob))
+        p.exec_image_to_grid()
+        launch = p.autotune(reps=2) if tune else p.get_launch()
+        t = time_steps(p, steps, warmup)
+        n_vis = p.gridded_visibilities
+    return {"value": n_vis / (t["ms_per_step"] / 1e3) / 1e6, "unit": "Mvis/s", "ms_per_step": t["ms_per_step"],
+            "gridder_ms": t["gridder_ms"], "degridder_ms": t["degridder_ms"], "launch": launch}
 
 
 def run_cycle(args):
@@ -222,9 +273,13 @@ def run_cycle(args):
     cd_ms = float(stats["deconv_ms"].sum())
     model_digest = hashlib.sha1(np.ascontiguousarray(x, dtype=np.float32).tobytes()).hexdigest()[:16]
     log("model image: sha1 %s, sum %.9g, max %.9g" % (model_digest, float(np.sum(x, dtype=np.float64)), float(x.max())))
+    vs_fp64 = compare_cycle_fixture(x, stats)
+    if vs_fp64:
+        log("against the FP64 CPU reconstruction: model max rel err %.3g, L2 rel err %.3g"
+            % (vs_fp64["model_max_rel"], vs_fp64["model_l2_rel"]))
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_oracle:
         # oracle costs on bounded samples of the same work: FastSerialCD
         # iterations on the full image (same PSF cut) and the FP64 grid +
         # degrid rate on a stride sample of subgrids
@@ -267,9 +322,10 @@ def run_cycle(args):
             "unit": "s",
             "cores": threads,
             "kind": "port",
-            "sample": "FastSerialCD: %d iterations on the 3072^2 image (%.1f ms each, 1 thread); grid + degrid: %d "
-                      "visibilities (%.2f us each, %d threads); scaled to %d iterations and %d x %d visibilities"
-                      % (k, t_iter * 1e3, nv, t_vis * 1e6, threads, total_iters, len(stats), n_vis),
+            "sample": "FastSerialCD: %d iterations on the 3072^2 image (%.1f ms each, argmax scan on %d threads); "
+                      "grid + degrid: %d visibilities (%.2f us each, %d threads); scaled to %d iterations and %d x %d "
+                      "visibilities" % (k, t_iter * 1e3, threads, nv, t_vis * 1e6, threads, total_iters, len(stats),
+                                        n_vis),
         }
     plan.close()
     out = {
@@ -300,8 +356,56 @@ def run_cycle(args):
                     ("lambda_", "abs_max_diff", "iterations", "converged", "gradient_ms", "deconv_ms", "imaging_ms")}
                    for s in stats],
         "cpu_baseline": cpu,
+        "vs_fp64_reconstruction": vs_fp64,
     }
     print(json.dumps(out), flush=True)
+
+
+CYCLE_FIXTURE = os.path.join(ROOT, "tests", "golden", "cycle_large.npz")
+
+
+def compare_cycle_fixture(x, stats):
+    """The GPU reconstruction against the FP64 CPU one of the same workload and
+    settings (tests/golden/cycle_large.npz, written by make_cycle_fixture.py
+    with the oracle gridding and degridding throughout): per-cycle figures and
+    the final model image's relative error.  None without the fixture."""
+    try:
+        f = np.load(CYCLE_FIXTURE, allow_pickle=False)
+    except OSError:
+        return None
+    ref_stats = f["stats"]
+    G = int(f["grid_size"])
+    if x.shape != (G, G):
+        return None
+    ref = np.zeros(G * G, dtype=np.float32)
+    ref[f["model_index"]] = f["model_value"]
+    ref = ref.reshape(G, G)
+    n = min(len(stats), len(ref_stats))
+    per_cycle = []
+    for k in range(n):
+        a, b = stats[k], ref_stats[k]
+        per_cycle.append({
+            "iterations": [int(a["iterations"]), int(b["iterations"])],
+            "lambda_rel": float(abs(a["lambda_"] - b["lambda_"]) / abs(b["lambda_"])) if b["lambda_"] else None,
+            "abs_max_diff_rel": float(abs(a["abs_max_diff"] - b["abs_max_diff"]) / abs(b["abs_max_diff"]))
+            if b["abs_max_diff"] else None,
+            "max_dirty_rel": float(abs(a["max_dirty"] - b["max_dirty"]) / abs(b["max_dirty"])) if b["max_dirty"] else None,
+        })
+    d = x.astype(np.float64) - ref
+    nz_gpu, nz_ref = x != 0, ref != 0
+    return {
+        "fixture": os.path.relpath(CYCLE_FIXTURE, ROOT),
+        "cycles_compared": n,
+        "cycles": [len(stats), len(ref_stats)],
+        "per_cycle": per_cycle,
+        "model_max_rel": float(np.abs(d).max() / np.abs(ref).max()),
+        "model_l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+        "model_sum": [float(x.sum(dtype=np.float64)), float(ref.sum(dtype=np.float64))],
+        "nonzero_pixels": [int(nz_gpu.sum()), int(nz_ref.sum()), int((nz_gpu & nz_ref).sum())],
+        "reference_sha1": str(f["digest"])[:16],
+        "reference_seconds": float(f["seconds"]),
+        "reference_threads": int(f["threads"]),
+    }
 
 
 def run_cd(args):
@@ -373,21 +477,31 @@ def run_batch(args):
         batch.launch()
     batch.sync()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # per-instance error on a stride sample of its subgrids: the replayed
+    # batch's degridded visibilities of those subgrids (a visibility depends
+    # only on its own subgrid), and GridW of the sample on a plan with the
+    # instance's tuned launch shape
     errors = []
     for ob, part, p in zip(obs_list, parts, plans):
         c = ob.c
-        oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize,
-                              c.MaxTimestepsPerSubgrid, float(c.ScaleArcSec), c.WLayerCount, c.WStep)
+        oc = oracle_constants(O, c)
         meta = np.ascontiguousarray(part.subgrids[:: max(1, part.total // 24)])
+        vis = p.download_model_vis()
+        model = p.download_model_grid().astype(np.complex128)
+        ref_v = O.degridw(oc, meta, model, ob.uvw, ob.freq)
+        launch = p.get_launch()
         with p9idg.Plan(c, kernel=args.kernel) as q:
+            q.set_launch(launch["queue"], launch["grid_chunk"], launch["degrid_chunk"], launch["workgroups"])
             q.set_observation(ob.uvw, ob.freq, meta)
             q.upload_vis(ob.vis)
             q.exec_grid(p9idg.GRID_WSTACK)
             layers = q.download_grid(p9idg.GRID_WSTACK)
         ref = O.gridw(oc, meta, ob.vis.astype(np.complex128), ob.uvw, ob.freq)
-        errors.append({"instance": ob.name, "grid": c.GridSize, "subgrid": c.SubgridSize,
+        errors.append({"instance": ob.name, "grid": c.GridSize, "subgrid": c.SubgridSize, "channels": len(ob.freq),
                        "gridded_visibilities": int(part.subgrids["timeSampleCount"].sum()) * len(ob.freq),
-                       "gridw_layers_max_rel": float(np.abs(layers - ref).max() / np.abs(ref).max())})
+                       "sample_subgrids": int(len(meta)),
+                       "gridw_layers": rel_errs(layers, ref),
+                       "degridw_vis": rel_errs(vis, ref_v, covered_mask(meta, vis.shape))})
     batch.close()
     for p in plans:
         p.close()
@@ -417,7 +531,8 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "idg_mfma_bf16_large_hbm_traffi
 
 def profiled_traffic(kernel, direction):
     """HBM bytes per launch of the large workload's kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated) and the launch
+    shape it was measured at, or (None, None)."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -425,10 +540,14 @@ def profiled_traffic(kernel, direction):
                 if line.startswith("hbm_bytes_per_launch:"):
                     table = json.loads(line.split(":", 1)[1])
                     v = table.get("%s/%s" % (kernel, direction))
-                    return None if v is None else int(v)
-    except (OSError, ValueError):
-        return None
-    return None
+                    if v is None:
+                        return None, None
+                    if isinstance(v, dict):
+                        return int(v["bytes"]), v.get("launch")
+                    return int(v), None
+    except (OSError, ValueError, KeyError):
+        return None, None
+    return None, None
 
 
 def copy_probe(gib):
@@ -458,15 +577,14 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="first run a plain device-to-device copy of this many GiB (3 times): the "
                          "reference kernel for scaling HBM byte counters")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU-baseline sample")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle", "--no-cpu-baseline", dest="no_oracle", action="store_true",
+                    help="skip the FP64 oracle: no error_vs_oracle and no cpu_baseline (about 1.5 min of host time)")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparator kernels")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--workload", default="large", choices=["large", "batch", "cycle", "cd"])
     ap.add_argument("--cycles", type=int, default=10, help="major cycles of --workload cycle")
     ap.add_argument("--deterministic", action="store_true",
                     help="ordered subgrid sums (large and cycle workloads): run-to-run identical grids")
-    ap.add_argument("--full-accuracy", action="store_true",
-                    help="also compare the whole workload's dirty image, PSF and degridded visibilities with the FP64 oracle")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -494,14 +612,15 @@ def main():
     part = ob.partition()
     t_part = time.perf_counter() - t
     c = ob.c
-    S = c.SubgridSize
+    S, G, L = c.SubgridSize, c.GridSize, c.WLayerCount
     log("partition: %d subgrids in %.3f s" % (part.total, t_part))
 
     plan = p9idg.Plan(c, device=0, kernel=args.kernel)
     plan.set_deterministic(args.deterministic)
     plan.set_observation(ob.uvw, ob.freq, part)
     plan.upload_vis(ob.vis)
-    plan.upload_image(model_image(ob))
+    model = model_image(ob)
+    plan.upload_image(model)
     plan.exec_image_to_grid()  # ToUVW of the model (MajorCycle.cs:125-127)
     plan.sync()
     n_vis = plan.gridded_visibilities
@@ -514,55 +633,93 @@ def main():
         tuning = plan.autotune(reps=3)
         log("autotuned launch shape: %s" % tuning)
 
-    def step():
-        plan.exec_grid(p9idg.GRID_WSTACK)
-        plan.exec_degrid(p9idg.DEGRID_WSTACK)
-
-    for _ in range(args.warmup):
-        step()
-    plan.sync()
-    log("warmup done")
-    plan.timing_reset(True)
-    plan.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    plan.sync()
-    t1 = time.perf_counter()
-    g_ms, g_n, d_ms, d_n = plan.timing_totals()
-    plan.timing_reset(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    tm = time_steps(plan, args.steps, args.warmup)
+    ms_per_step = tm["ms_per_step"]
     value = n_vis / (ms_per_step / 1e3) / 1e6
-    avg_g = g_ms / max(g_n, 1)
-    avg_d = d_ms / max(d_n, 1)
-    log("step %.3f ms: gridder %.3f ms, degridder %.3f ms" % (ms_per_step, avg_g, avg_d))
+    avg_g, avg_d = tm["gridder_ms"], tm["degridder_ms"]
+    log("step %.3f ms: gridder %.3f ms, degridder %.3f ms, zero fills %.3f ms" % (ms_per_step, avg_g, avg_d,
+                                                                                 tm["zero_ms"]))
 
-    # layer FFT + image extraction, timed once (not part of the metric)
-    t = time.perf_counter()
-    plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+    # HBM-bound stages: ToImage of the 32 layers and ToUVW, HIP events over
+    # `steps` runs each (after the timed region; not part of the metric)
+    plan.timing_reset(True)
+    for _ in range(args.steps):
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        plan.exec_image_to_grid()
     plan.sync()
-    fft_wall = (time.perf_counter() - t) * 1e3
+    ti_ms, ti_n = plan.timing_stage(p9idg.STAGE_TO_IMAGE)
+    tu_ms, tu_n = plan.timing_stage(p9idg.STAGE_TO_UVW)
+    plan.timing_reset(False)
+    area, union = wstack_footprint(part.subgrids, G, S, L)
+    GG = G * G
+    fft_bytes = 4 * GG * 8  # an in-place G x G complex64 FFT counted as two read + write passes
+    stage_bytes = {
+        # the footprint boxes of the previous gridding, zeroed before each gridder launch
+        "zero": area * 8,
+        # zero the layer sum, sum the layer boxes, inverse FFT, real part / N into the shifted image
+        "to_image": GG * 8 + area * 8 + union * 8 + fft_bytes + GG * 8 + GG * 4,
+        # shifted image into a complex grid, forward FFT
+        "to_uvw": GG * 4 + GG * 8 + fft_bytes,
+    }
+    stage_ms = {"zero": tm["zero_ms"], "to_image": ti_ms / max(ti_n, 1), "to_uvw": tu_ms / max(tu_n, 1)}
+    hbm = {k: {"ms": stage_ms[k], "bytes": int(stage_bytes[k]),
+               "gb_s": stage_bytes[k] / (stage_ms[k] / 1e3) / 1e9 if stage_ms[k] > 0 else None}
+           for k in stage_bytes}
+    for k, v in hbm.items():
+        v["frac_of_peak"] = v["gb_s"] / HBM_PEAK_GBS if v["gb_s"] else None
+    log("hbm stages: " + ", ".join("%s %.3f ms %.0f GB/s" % (k, v["ms"], v["gb_s"] or 0) for k, v in hbm.items()))
+
+    # the timed plan (its launch shape and adder) on the whole workload, for the oracle
+    gpu = {}
+    if not args.no_oracle:
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        gpu["dirty"] = plan.download_image()
+        plan.set_psf_vis(None)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        gpu["psf"] = plan.download_image()
+        plan.upload_vis(ob.vis)
+        plan.upload_image(model)
+        plan.exec_image_to_grid()
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        gpu["vis"] = plan.download_model_vis()
+    model_grid = plan.download_model_grid()
+    plan.close()
+
+    # the FP32 vector-ALU kernels on the same step, each autotuned
+    vs_fp32 = None
+    if args.kernel.startswith("mfma") and not args.no_fp32:
+        vs_fp32 = {}
+        for k in ("fp32_sep", "fp32"):
+            vs_fp32[k] = comparator(ob, part, k, args.steps, args.warmup, not args.no_autotune)
+            log("%s: %.3f ms per step, %.0f Mvis/s" % (k, vs_fp32[k]["ms_per_step"], vs_fp32[k]["value"]))
+        best = max(vs_fp32, key=lambda k: vs_fp32[k]["value"])
+        vs_fp32["faster_fp32"] = best
+        vs_fp32["ratio"] = value / vs_fp32[best]["value"]
 
     flops_per_launch = 8.0 * S * S * n_vis  # complex MAC per pixel per visibility
     dom_name, dom_ms = ("gridder", avg_g) if avg_g >= avg_d else ("degridder", avg_d)
     achieved = flops_per_launch / (dom_ms / 1e3) / 1e12
     peak = PEAK_TFLOPS[args.kernel]
+    traffic, traffic_launch = profiled_traffic(args.kernel, dom_name)
     roofline = {
-        "bound": "mfma",
+        "bound": BOUND[args.kernel],
         "kernel": dom_name,
         "achieved": achieved,
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
-        "traffic": profiled_traffic(args.kernel, dom_name),
+        "traffic": traffic,
+        "traffic_launch": traffic_launch,
+        "flops_per_launch": flops_per_launch,
+        "kernel_ms": dom_ms,
     }
-    cpu, err, full = None, None, None
-    model_grid = plan.download_model_grid()
-    plan.close()
-    if not args.no_cpu_baseline:
-        cpu, err = cpu_baseline_and_error(args, ob, part, model_grid, args.kernel)
-    if args.full_accuracy:
-        full = full_accuracy(ob, part, model_grid, args.kernel)
+    cpu, err = None, None
+    if not args.no_oracle:
+        cpu, err = oracle_checks(ob, part, gpu, model_grid)
+        err["launch"] = tuning
+        err["deterministic_adder"] = bool(args.deterministic)
 
     out = {
         "metric": METRIC,
@@ -589,15 +746,17 @@ def main():
         "stages": {
             "gridder_ms": avg_g,
             "degridder_ms": avg_d,
-            "layer_ifft_and_image_ms": fft_wall,
+            "zero_fill_ms": tm["zero_ms"],
+            "to_image_ms": stage_ms["to_image"],
+            "to_uvw_ms": stage_ms["to_uvw"],
             "partition_ms": t_part * 1e3,
         },
+        "vs_fp32": vs_fp32,
+        "hbm": hbm,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "error_vs_oracle": err,
     }
-    if full is not None:
-        out["error_vs_oracle_full"] = full
     print(json.dumps(out), flush=True)
 
 

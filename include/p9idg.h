@@ -97,7 +97,11 @@ typedef enum p9idg_vis_type { P9IDG_C128 = 0, P9IDG_C64 = 1 } p9idg_vis_type;
 typedef enum p9idg_kernel {
     P9IDG_KERNEL_FP32 = 0,      /* per-pixel phasors on the FP32 vector ALUs */
     P9IDG_KERNEL_MFMA_BF16 = 1, /* split hi/lo BF16 MFMA contraction, FP32 accumulate */
-    P9IDG_KERNEL_MFMA_FP16 = 2  /* split hi/lo FP16 MFMA contraction, FP32 accumulate */
+    P9IDG_KERNEL_MFMA_FP16 = 2, /* split hi/lo FP16 MFMA contraction, FP32 accumulate */
+    P9IDG_KERNEL_FP32_SEP = 3   /* the MFMA kernels' separable phasors (2 S per visibility)
+                                   contracted with FP32 FMAs on the vector ALUs; gridder for
+                                   S = 16, 32, 48, 64 and degridder for S = 16, 32, the
+                                   per-pixel FP32 form for other sizes */
 } p9idg_kernel;
 
 /* Where p9idg_exec_grid puts the subgrids. */
@@ -264,8 +268,25 @@ int p9idg_last_timings(p9idg_plan* plan, float* grid_ms, float* degrid_ms, float
 int p9idg_timing_reset(p9idg_plan* plan, int32_t collect);
 int p9idg_timing_totals(p9idg_plan* plan, double* grid_ms, int64_t* grid_launches,
                         double* degrid_ms, int64_t* degrid_launches);
+/* Stages timed by the collected spans.  ZERO covers the zero fills of the
+ * gridder target before each gridder launch (the footprint of the previous
+ * gridding, or the whole buffer when its contents are unknown) and of the
+ * model visibilities before a degridder launch (when needed); TO_IMAGE is
+ * p9idg_exec_grid_to_image (layer sum, inverse FFT, image extraction);
+ * TO_UVW is p9idg_exec_image_to_grid (shift + real-to-complex copy, forward
+ * FFT).  No counterpart in the reference (its callers use stopwatches). */
+typedef enum p9idg_stage {
+    P9IDG_STAGE_GRIDDER = 0,
+    P9IDG_STAGE_DEGRIDDER = 1,
+    P9IDG_STAGE_ZERO = 2,
+    P9IDG_STAGE_TO_IMAGE = 3,
+    P9IDG_STAGE_TO_UVW = 4
+} p9idg_stage;
+/* Total milliseconds and number of the collected spans of one stage. */
+int p9idg_timing_stage(p9idg_plan* plan, int32_t stage, double* ms_total, int64_t* spans);
 /* Number of visibilities covered by the partition (sum of count * C). */
 int p9idg_gridded_visibilities(p9idg_plan* plan, int64_t* n);
+
 
 /* ---- Batches: several observations in one persistent launch -------------- */
 /* The plans keep their observations and buffers; a batch groups them by

@@ -353,7 +353,7 @@ int orc_partition(const orc_constants* c, const double* uvw, int32_t B, int32_t 
     if (status != ORC_OK) return status;
     if (out) {
         if ((int64_t)result.size() > capacity) return ORC_ERR_CAPACITY;
-        std::memcpy(out, result.data(), result.size() * sizeof(orc_subgrid));
+        if (!result.empty()) std::memcpy(out, result.data(), result.size() * sizeof(orc_subgrid));
     }
     return ORC_OK;
 }

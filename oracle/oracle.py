@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# P9IDG_ORACLE_LIB selects another build (the sanitizer build _build/asan/liboracle.so)
+_LIB_PATH = os.environ.get("P9IDG_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 SUBGRID_DTYPE = np.dtype(
     [

@@ -45,10 +45,13 @@ struct Pixel {
 };
 
 // FastSerialCD.cs:138-172: per row the first x with the largest |x - prox|,
-// then the first row with the largest of those.
+// then the first row with the largest of those.  The rows are scanned in
+// parallel (each row's best is independent of the others) and the rows'
+// bests are then compared in row order, as the reference's serial loop does.
 Pixel abs_max(const float* amap, const float* x, const float* g, int32_t H, int32_t W, float lambda,
               float alpha) {
-    Pixel best{-1, -1, 0.f, 0.f};
+    std::vector<Pixel> rows((size_t)std::max(H, 0));
+#pragma omp parallel for schedule(static) num_threads(orc_get_threads())
     for (int32_t y = 0; y < H; y++) {
         Pixel row{-1, -1, 0.f, 0.f};
         for (int32_t xx = 0; xx < W; xx++) {
@@ -59,8 +62,11 @@ Pixel abs_max(const float* amap, const float* x, const float* g, int32_t H, int3
             const float diff = old - tmp;
             if (row.diff < std::fabs(diff)) row = Pixel{y, xx, std::fabs(diff), tmp};
         }
-        if (best.diff < row.diff) best = row;
+        rows[y] = row;
     }
+    Pixel best{-1, -1, 0.f, 0.f};
+    for (int32_t y = 0; y < H; y++)
+        if (best.diff < rows[y].diff) best = rows[y];
     return best;
 }
 

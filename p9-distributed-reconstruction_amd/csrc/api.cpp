@@ -141,7 +141,7 @@ struct p9idg_plan {
     // (p9idg_timing_reset), for averages over a timed region.
     struct Span {
         hipEvent_t start, stop;
-        int kind;  // 0 grid, 1 degrid
+        int kind;  // p9idg_stage
     };
     std::vector<hipEvent_t> event_pool;
     std::vector<Span> spans;
@@ -570,12 +570,15 @@ int do_exec_grid(p9idg_plan* p, int mode) {
         a.scratch = p->d_scratch;
     }
     if (int st_ = record(p, 0)) return st_;
+    if (int st = span_begin(p, P9IDG_STAGE_ZERO)) return st;
     if (int st = zero_target(p, mode, target, n)) return st;
+    if (int st = span_end(p)) return st;
     if (int st_ = record(p, 1)) return st_;
-    if (int st = span_begin(p, 0)) return st;
+    if (int st = span_begin(p, P9IDG_STAGE_GRIDDER)) return st;
     hipError_t e;
-    if (p->kernel == P9IDG_KERNEL_FP32)
-        e = launch_grid_fp32(a, p->c.subgrid_size, launch_cfg(p, true), p->stream);
+    if (p->kernel == P9IDG_KERNEL_FP32 || p->kernel == P9IDG_KERNEL_FP32_SEP)
+        e = launch_grid_fp32(a, p->c.subgrid_size, p->kernel == P9IDG_KERNEL_FP32_SEP, launch_cfg(p, true),
+                             p->stream);
     else
         e = launch_grid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p, true), p->stream);
     if (e != hipSuccess) return hip_fail(e, "gridder launch");
@@ -601,13 +604,18 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     a.nitems = p->nitems_degrid;
     // Gridder.cs:96: uncovered timesteps stay 0; only the degridder writes
     // this buffer, and always the same (covered) entries
-    if (p->capturing || !p->model_vis_clean) P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));
+    if (p->capturing || !p->model_vis_clean) {
+        if (int st = span_begin(p, P9IDG_STAGE_ZERO)) return st;
+        P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));
+        if (int st = span_end(p)) return st;
+    }
     p->model_vis_clean = !p->capturing;
     if (int st_ = record(p, 3)) return st_;
-    if (int st = span_begin(p, 1)) return st;
+    if (int st = span_begin(p, P9IDG_STAGE_DEGRIDDER)) return st;
     hipError_t e;
-    if (p->kernel == P9IDG_KERNEL_FP32)
-        e = launch_degrid_fp32(a, p->c.subgrid_size, launch_cfg(p, false), p->stream);
+    if (p->kernel == P9IDG_KERNEL_FP32 || p->kernel == P9IDG_KERNEL_FP32_SEP)
+        e = launch_degrid_fp32(a, p->c.subgrid_size, p->kernel == P9IDG_KERNEL_FP32_SEP, launch_cfg(p, false),
+                               p->stream);
     else
         e = launch_degrid_mfma(a, p->c.subgrid_size, p->kernel, launch_cfg(p, false), p->stream);
     if (e != hipSuccess) return hip_fail(e, "degridder launch");
@@ -648,6 +656,7 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
     const int G = p->c.grid_size;
     const double inv_n = 1.0 / (double)p->c.visibilities_count;
     if (int st_ = record(p, 5)) return st_;
+    if (int st = span_begin(p, P9IDG_STAGE_TO_IMAGE)) return st;
     if (mode == P9IDG_GRID_WSTACK) {
         if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
         if (int st = layers_image(p, inv_n, 1)) return st;
@@ -658,6 +667,7 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
                              reinterpret_cast<hipfftComplex*>(p->d_grid), HIPFFT_BACKWARD));
         P9_HIP(launch_layers_to_image(p->d_grid, 1, G, inv_n, 1, p->d_image, p->stream));
     }
+    if (int st = span_end(p)) return st;
     if (int st_ = record(p, 6)) return st_;
     p->timed_fft = true;
     return P9IDG_OK;
@@ -667,10 +677,11 @@ int do_image_to_grid(p9idg_plan* p) {
     if (int st = check_obs(p)) return st;
     DeviceGuard dg(p->device);
     if (int st = ensure_fft_one(p)) return st;
+    if (int st = span_begin(p, P9IDG_STAGE_TO_UVW)) return st;
     P9_HIP(launch_image_to_grid(p->d_image, p->c.grid_size, 1, p->d_model_grid, p->stream));
     P9_FFT(hipfftExecC2C(p->fft_one, reinterpret_cast<hipfftComplex*>(p->d_model_grid),
                          reinterpret_cast<hipfftComplex*>(p->d_model_grid), HIPFFT_FORWARD));
-    return P9IDG_OK;
+    return span_end(p);
 }
 
 int do_residual(p9idg_plan* p) {
@@ -793,12 +804,14 @@ int p9idg_partition(const p9idg_constants* c, const double* uvw, int32_t nr_base
     if (st == P9IDG_ERR_PLACE) return fail(st, kMsgPlace);
     if (st == P9IDG_ERR_SUBGRID_RANGE) return fail(st, kMsgSubgridRange);
     if (st) return fail(st, "partition failed");
-    if (baseline_offsets) std::memcpy(baseline_offsets, offs.data(), offs.size() * sizeof(int64_t));
+    // (memcpy's source must not be NULL even for 0 bytes: empty vectors are skipped)
+    if (baseline_offsets && !offs.empty())
+        std::memcpy(baseline_offsets, offs.data(), offs.size() * sizeof(int64_t));
     if (out) {
         if ((int64_t)subs.size() > capacity)
             return fail(P9IDG_ERR_CAPACITY, "output capacity " + std::to_string(capacity) +
                                                 " < " + std::to_string(subs.size()) + " subgrids");
-        std::memcpy(out, subs.data(), subs.size() * sizeof(p9idg_subgrid));
+        if (!subs.empty()) std::memcpy(out, subs.data(), subs.size() * sizeof(p9idg_subgrid));
     }
     return P9IDG_OK;
 }
@@ -911,7 +924,7 @@ int p9idg_destroy(p9idg_plan* p) {
 int p9idg_set_kernel(p9idg_plan* p, int32_t kernel) {
     if (int st = check_plan(p)) return st;
     if (kernel != P9IDG_KERNEL_FP32 && kernel != P9IDG_KERNEL_MFMA_BF16 &&
-        kernel != P9IDG_KERNEL_MFMA_FP16)
+        kernel != P9IDG_KERNEL_MFMA_FP16 && kernel != P9IDG_KERNEL_FP32_SEP)
         return fail(P9IDG_ERR_ARGUMENT, "unknown kernel");
     if (p->kernel != kernel) drop_graph(p);  // a captured step holds the old kernel
     p->kernel = kernel;
@@ -1513,6 +1526,25 @@ int p9idg_timing_reset(p9idg_plan* p, int32_t collect) {
     return P9IDG_OK;
 }
 
+int p9idg_timing_stage(p9idg_plan* p, int32_t stage, double* ms_total, int64_t* spans) {
+    if (int st = check_plan(p)) return st;
+    if (stage < P9IDG_STAGE_GRIDDER || stage > P9IDG_STAGE_TO_UVW) return fail(P9IDG_ERR_ARGUMENT, "unknown stage");
+    DeviceGuard dg(p->device);
+    P9_HIP(hipStreamSynchronize(p->stream));
+    double t = 0.0;
+    int64_t n = 0;
+    for (const auto& sp : p->spans) {
+        if (sp.kind != stage) continue;
+        float ms = 0.f;
+        P9_HIP(hipEventElapsedTime(&ms, sp.start, sp.stop));
+        t += ms;
+        n++;
+    }
+    if (ms_total) *ms_total = t;
+    if (spans) *spans = n;
+    return P9IDG_OK;
+}
+
 int p9idg_timing_totals(p9idg_plan* p, double* grid_ms, int64_t* grid_launches,
                         double* degrid_ms, int64_t* degrid_launches) {
     if (int st = check_plan(p)) return st;
@@ -1521,6 +1553,7 @@ int p9idg_timing_totals(p9idg_plan* p, double* grid_ms, int64_t* grid_launches,
     double t[2] = {0.0, 0.0};
     int64_t n[2] = {0, 0};
     for (const auto& sp : p->spans) {
+        if (sp.kind > P9IDG_STAGE_DEGRIDDER) continue;
         float ms = 0.f;
         P9_HIP(hipEventElapsedTime(&ms, sp.start, sp.stop));
         t[sp.kind] += ms;

@@ -1084,7 +1084,8 @@ hipError_t launch_batch_kernel(K kernel, const BatchArgs& b, int workgroups, hip
 
 template <int S>
 hipError_t batch_dispatch(bool grid, const BatchArgs& b, int kernel, int wg, hipStream_t st) {
-    if (kernel == P9IDG_KERNEL_FP32) return launch_fp32_batch(grid, b, S, wg, st);
+    if (kernel == P9IDG_KERNEL_FP32 || kernel == P9IDG_KERNEL_FP32_SEP)
+        return launch_fp32_batch(grid, kernel == P9IDG_KERNEL_FP32_SEP, b, S, wg, st);
     if (kernel == P9IDG_KERNEL_MFMA_FP16)
         return grid ? launch_batch_kernel(dev::grid_mfma_batch_kernel<S, dev::PrecF16>, b, wg, st)
                     : launch_batch_kernel(dev::degrid_mfma_batch_kernel<S, dev::PrecF16>, b, wg, st);

@@ -131,15 +131,16 @@ struct BatchArgs {
 };
 
 // Launchers (kernels_*.hip); S is the subgrid size.
-hipError_t launch_grid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st);
-hipError_t launch_degrid_fp32(const KernelArgs& a, int S, const LaunchCfg& cfg, hipStream_t st);
+// sep: the separable FP32 form (P9IDG_KERNEL_FP32_SEP) where S allows it
+hipError_t launch_grid_fp32(const KernelArgs& a, int S, bool sep, const LaunchCfg& cfg, hipStream_t st);
+hipError_t launch_degrid_fp32(const KernelArgs& a, int S, bool sep, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
                             hipStream_t st);
 hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
                               hipStream_t st);
 hipError_t launch_reset_counter(int32_t* counter, hipStream_t st);
 hipError_t launch_rtab(float* rtab, int S, float image_size, hipStream_t st);
-hipError_t launch_fp32_batch(bool grid, const BatchArgs& b, int S, int workgroups, hipStream_t st);
+hipError_t launch_fp32_batch(bool grid, bool sep, const BatchArgs& b, int S, int workgroups, hipStream_t st);
 // kernel: p9idg_kernel; workgroups 0 = CUs x occupancy
 hipError_t launch_grid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st);
 hipError_t launch_degrid_batch(const BatchArgs& b, int S, int kernel, int workgroups, hipStream_t st);

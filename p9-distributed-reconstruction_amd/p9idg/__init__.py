@@ -26,4 +26,9 @@ from ._lib import (  # noqa: F401
     GRID_WSUM,
     SRC_RESIDUAL,
     SRC_VIS,
+    STAGE_DEGRIDDER,
+    STAGE_GRIDDER,
+    STAGE_TO_IMAGE,
+    STAGE_TO_UVW,
+    STAGE_ZERO,
 )

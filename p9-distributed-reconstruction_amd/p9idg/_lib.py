@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.dirname(PKG_DIR)
+# P9IDG_LIB_DIR selects another build of the two libraries (the sanitizer
+# build in _build/asan); by default the in-tree ones next to the package
+LIB_DIR = os.environ.get("P9IDG_LIB_DIR") or os.path.dirname(PKG_DIR)
 IDG_LIB_PATH = os.path.join(LIB_DIR, "libp9idg.so")
 OBS_LIB_PATH = os.path.join(LIB_DIR, "libp9obs.so")
 
@@ -32,7 +34,9 @@ C64 = 1
 KERNEL_FP32 = 0
 KERNEL_MFMA_BF16 = 1
 KERNEL_MFMA_FP16 = 2
+KERNEL_FP32_SEP = 3
 
+STAGE_GRIDDER, STAGE_DEGRIDDER, STAGE_ZERO, STAGE_TO_IMAGE, STAGE_TO_UVW = 0, 1, 2, 3, 4
 GRID_PLAIN = 0
 GRID_WSTACK = 1
 GRID_WSUM = 2
@@ -142,6 +146,7 @@ IDG_SIGNATURES = {
     "p9idg_last_timings": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "p9idg_timing_reset": [_vp, _i32],
     "p9idg_timing_totals": [_vp, ctypes.POINTER(_f64), _pi64, ctypes.POINTER(_f64), _pi64],
+    "p9idg_timing_stage": [_vp, _i32, ctypes.POINTER(_f64), _pi64],
     "p9idg_gridded_visibilities": [_vp, _pi64],
     "p9idg_batch_create": [_vp, _i32, ctypes.POINTER(_vp)],
     "p9idg_batch_destroy": [_vp],

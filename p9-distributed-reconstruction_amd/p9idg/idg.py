@@ -38,7 +38,8 @@ __all__ = [
     "SUBGRID_DTYPE",
 ]
 
-_KERNELS = {"fp32": _lib.KERNEL_FP32, "mfma_bf16": _lib.KERNEL_MFMA_BF16, "mfma_fp16": _lib.KERNEL_MFMA_FP16}
+_KERNELS = {"fp32": _lib.KERNEL_FP32, "fp32_sep": _lib.KERNEL_FP32_SEP, "mfma_bf16": _lib.KERNEL_MFMA_BF16,
+            "mfma_fp16": _lib.KERNEL_MFMA_FP16}
 
 
 class GriddingConstants:
@@ -396,6 +397,13 @@ class Plan:
         ng, nd = ctypes.c_int64(), ctypes.c_int64()
         check(_lib.idg().p9idg_timing_totals(self._h, ctypes.byref(g), ctypes.byref(ng), ctypes.byref(d), ctypes.byref(nd)))
         return g.value, ng.value, d.value, nd.value
+
+    def timing_stage(self, stage):
+        """(total ms, spans) of one collected stage (p9idg_stage: gridder,
+        degridder, zero fills, ToImage, ToUVW)."""
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        check(_lib.idg().p9idg_timing_stage(self._h, int(stage), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
 
     @property
     def gridded_visibilities(self):

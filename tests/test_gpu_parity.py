@@ -1,6 +1,6 @@
 """GPU parity: every gridding entry point of libp9idg (through the C ABI)
-against the FP64 oracle on the same seeded inputs, for the FP32 vector kernel
-and the split BF16 / FP16 MFMA kernels.
+against the FP64 oracle on the same seeded inputs, for the per-pixel and the
+separable FP32 vector kernels and the split BF16 / FP16 MFMA kernels.
 
 Tolerance: max |gpu - oracle| / max |oracle| <= 1e-5 for grids, images and
 degridded visibilities (the accuracy bar of the split-precision form).  The
@@ -17,7 +17,7 @@ from conftest import oracle_constants, rel_err
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = ["fp32", "mfma_bf16", "mfma_fp16"]
+KERNELS = ["fp32", "fp32_sep", "mfma_bf16", "mfma_fp16"]
 TOL = 1e-5
 
 
@@ -539,7 +539,7 @@ def test_major_cycle_step_eager_graph_and_oracle(small, O, kernel):
             assert rel_err(img, eager) <= 1e-6, rep
 
 
-@pytest.mark.parametrize("kernels", [("mfma_bf16",), ("fp32",), ("mfma_fp16", "fp32")])
+@pytest.mark.parametrize("kernels", [("mfma_bf16",), ("fp32",), ("mfma_fp16", "fp32"), ("fp32_sep", "mfma_bf16")])
 def test_batched_uneven_instances_match_oracle(O, kernels):
     """Observations with subgrid sizes 24, 32 and 48 in one batch: one
     persistent launch per (size, kernel) group, eager and replayed from a
