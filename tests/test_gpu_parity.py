@@ -615,3 +615,35 @@ def test_batched_uneven_instances_match_oracle(O, kernels):
     finally:
         for p in plans:
             p.close()
+
+
+@pytest.mark.parametrize("kernel", ["mfma_bf16", "fp32_sep"])
+def test_autotuned_launch_shape_matches_oracle(O, kernel):
+    """p9idg_autotune picks a shape among its candidates and leaves the plan on
+    it; GridW and DeGridW at that shape (atomic and ordered adder) equal the
+    oracle."""
+    ob = obs.custom(G=512, S=32, K=8, L=4, n_ant=20, T=100, C=8, seed=88, core_diam=2500.0, noise=0.01)
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    ref_l = O.gridw(oc, part.subgrids, ob.vis, ob.uvw, ob.freq)
+    model = ref_l.sum(axis=0)
+    ref_v = O.degridw(oc, part.subgrids, model, ob.uvw, ob.freq)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.upload_model_grid(model.astype(np.complex64))
+        tuned = plan.autotune(reps=1)
+        print(kernel, "autotuned", tuned)
+        assert tuned["grid_ms"] > 0 and tuned["degrid_ms"] > 0
+        assert tuned["grid_chunk"] in (0, 512, 1024, 2048, 4096) and tuned["degrid_chunk"] in (0, 512, 1024, 2048, 4096)
+        assert plan.get_launch() == {k: tuned[k] for k in ("queue", "grid_chunk", "degrid_chunk", "workgroups")}
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        assert rel_err(plan.download_grid(p9idg.GRID_WSTACK), ref_l) <= TOL
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        assert rel_err(plan.download_model_vis(), ref_v) <= TOL
+        plan.set_deterministic(True)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        det = plan.download_grid(p9idg.GRID_WSTACK)
+        assert rel_err(det, ref_l) <= TOL
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        assert np.array_equal(plan.download_grid(p9idg.GRID_WSTACK), det)
