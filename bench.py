@@ -550,22 +550,6 @@ def profiled_traffic(kernel, direction):
     return None, None
 
 
-def copy_probe(gib):
-    """A plain copy kernel of known size, profiled beside the gridder to turn
-    FETCH_SIZE / WRITE_SIZE counter readings into bytes."""
-    import torch
-
-    n = int(gib * (1 << 30)) // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    log("copy probe: %d bytes x 3 launches" % (4 * n))
-    del src, dst
-    torch.cuda.empty_cache()
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -574,9 +558,6 @@ def main():
     ap.add_argument("--kernel", default="mfma_bf16", choices=sorted(PEAK_TFLOPS))
     ap.add_argument("--launch", default=None, metavar="QUEUE,GRID_CHUNK,DEGRID_CHUNK,WORKGROUPS",
                     help="fixed launch shape instead of the autotuner's (e.g. 1,0,0,256)")
-    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
-                    help="first run a plain device-to-device copy of this many GiB (3 times): the "
-                         "reference kernel for scaling HBM byte counters")
     ap.add_argument("--no-oracle", "--no-cpu-baseline", dest="no_oracle", action="store_true",
                     help="skip the FP64 oracle: no error_vs_oracle and no cpu_baseline (about 1.5 min of host time)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparator kernels")
@@ -593,8 +574,6 @@ def main():
               "(requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
 
-    if args.copy_probe > 0:
-        copy_probe(args.copy_probe)
     if args.workload == "batch":
         return run_batch(args)
     if args.workload == "cycle":
