@@ -287,6 +287,25 @@ int p9idg_timing_stage(p9idg_plan* plan, int32_t stage, double* ms_total, int64_
 /* Number of visibilities covered by the partition (sum of count * C). */
 int p9idg_gridded_visibilities(p9idg_plan* plan, int64_t* n);
 
+/* ---- Allocation-free gridding --------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): p9idg_exec_grid's gridder -> subgrid FFT -> adder for
+ * `mode` (P9IDG_GRID_PLAIN / WSTACK / WSUM) with `kernel`, without a plan.
+ * Inputs: the subgrid metadata and frequencies in host memory (read during
+ * the call), uvw double[B][T][3] and visibilities complex64[B][T][C] in device
+ * memory; output d_out complex64 [L][G][G] (WSTACK) or [G][G], zeroed first.
+ * The work-queue counter, work items, metadata, wavenumbers and tables live in
+ * a fixed 4 MiB device array per GPU (allocated with the library's code
+ * object, held until the process ends): at most 65536 subgrids and 4096
+ * channels (P9IDG_ERR_CAPACITY beyond).  Calls no hipMalloc / hipFree.  The
+ * work runs on `stream` (a hipStream_t; NULL: the null stream); the copies of
+ * the host arrays wait for the work already on the stream.  Every call on a
+ * GPU shares its array: calls that overlap on one GPU (other streams, other
+ * threads) race.  Atomic adder (no deterministic mode). */
+int p9idg_grid_noalloc(const p9idg_constants* c, int32_t kernel, int32_t mode, const p9idg_subgrid* meta,
+                       int32_t nr_subgrids, const double* d_uvw, int32_t nr_baselines, int32_t nr_timesteps,
+                       const double* freq, int32_t nr_channels, const void* d_vis, void* d_out, void* stream);
 
 /* ---- Batches: several observations in one persistent launch -------------- */
 /* The plans keep their observations and buffers; a batch groups them by

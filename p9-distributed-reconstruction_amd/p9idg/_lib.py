@@ -147,6 +147,8 @@ IDG_SIGNATURES = {
     "p9idg_timing_reset": [_vp, _i32],
     "p9idg_timing_totals": [_vp, ctypes.POINTER(_f64), _pi64, ctypes.POINTER(_f64), _pi64],
     "p9idg_timing_stage": [_vp, _i32, ctypes.POINTER(_f64), _pi64],
+    "p9idg_grid_noalloc": [ctypes.POINTER(Constants), _i32, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp,
+                           _vp],
     "p9idg_gridded_visibilities": [_vp, _pi64],
     "p9idg_batch_create": [_vp, _i32, ctypes.POINTER(_vp)],
     "p9idg_batch_destroy": [_vp],
