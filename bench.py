@@ -691,6 +691,8 @@ def main():
         "frac": achieved / peak,
         "traffic": traffic,
         "traffic_launch": traffic_launch,
+        "traffic_at_timed_launch": bool(traffic_launch) and all(
+            traffic_launch.get(k) == tuning.get(k) for k in ("queue", "grid_chunk", "degrid_chunk", "workgroups")),
         "flops_per_launch": flops_per_launch,
         "kernel_ms": dom_ms,
     }
