@@ -12,7 +12,16 @@ pixels (flat index, float32 value), and its SHA-1 digest.  bench.py
 
 Run from the repository root after building (python __graft_entry__.py build):
     python tests/golden/make_cycle_fixture.py --threads 8
-It takes about an hour on 8 cores."""
+It takes about an hour on 8 cores.
+
+--perturb EPS multiplies every visibility by (1 + EPS z), z complex standard
+normal (seed 1), before the reconstruction, and --compare FIXTURE prints the
+per-cycle and final-model differences against a fixture: with EPS about the
+FP32 rounding error this measures how far the deconvolution alone carries a
+difference of that size, e.g.
+
+    python tests/golden/make_cycle_fixture.py --cycles 4 --perturb 1e-7 \
+        --out /tmp/cycle_perturbed.npz --compare tests/golden/cycle_large.npz"""
 import argparse
 import hashlib
 import os
@@ -37,6 +46,8 @@ def main():
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--cycles", type=int, default=SETTINGS["cycles"])
     ap.add_argument("--out", default=OUT)
+    ap.add_argument("--perturb", type=float, default=0.0, help="relative visibility perturbation")
+    ap.add_argument("--compare", default=None, help="fixture to compare the result with")
     args = ap.parse_args()
     O.set_threads(args.threads)
     t = time.perf_counter()
@@ -46,8 +57,13 @@ def main():
     oc = O.make_constants(c.VisibilitiesCount, c.GridSize, c.SubgridSize, c.KernelSize, c.MaxTimestepsPerSubgrid,
                           float(c.ScaleArcSec), c.WLayerCount, c.WStep)
     print("workload: %d subgrids, %.1f s" % (part.total, time.perf_counter() - t), flush=True)
+    vis = ob.vis.astype(np.complex128)
+    if args.perturb:
+        rng = np.random.default_rng(1)
+        z = rng.standard_normal(vis.shape) + 1j * rng.standard_normal(vis.shape)
+        vis = vis * (1.0 + args.perturb * z)
     t = time.perf_counter()
-    x, stats = O.reconstruct_serial_cd(oc, part.subgrids, ob.vis.astype(np.complex128), None, ob.uvw, ob.freq,
+    x, stats = O.reconstruct_serial_cd(oc, part.subgrids, vis, None, ob.uvw, ob.freq,
                                        SETTINGS["psf_cut"], args.cycles, SETTINGS["lambda"], SETTINGS["alpha"],
                                        SETTINGS["iterations"], SETTINGS["epsilon"])
     secs = time.perf_counter() - t
@@ -62,6 +78,28 @@ def main():
                         threads=np.int32(args.threads), **{"setting_" + k: np.float64(v) for k, v in SETTINGS.items()})
     print("%d cycles in %.0f s; model: %d nonzero pixels, sum %.9g, sha1 %s -> %s"
           % (len(stats), secs, idx.size, float(x.sum(dtype=np.float64)), digest[:16], args.out), flush=True)
+    if args.compare:
+        compare(x, stats, args.compare)
+
+
+def compare(x, stats, path):
+    """Prints the relative differences of the per-cycle figures and of the
+    model image against the fixture at path (cycles present in both)."""
+    f = np.load(path, allow_pickle=False)
+    ref_stats = f["stats"]
+    for k in range(min(len(stats), len(ref_stats))):
+        a, b = stats[k], ref_stats[k]
+        print("cycle %d vs fixture: lambda %.3g, |dx|max %.3g, max dirty %.3g (relative), iterations %d / %d"
+              % (k, abs(a["lambda_"] - b["lambda_"]) / abs(b["lambda_"]),
+                 abs(a["abs_max_diff"] - b["abs_max_diff"]) / abs(b["abs_max_diff"]),
+                 abs(a["max_dirty"] - b["max_dirty"]) / abs(b["max_dirty"]), a["iterations"], b["iterations"]))
+    if len(stats) == len(ref_stats):
+        G = int(f["grid_size"])
+        ref = np.zeros(G * G, np.float32)
+        ref[f["model_index"]] = f["model_value"]
+        d = x.astype(np.float64).ravel() - ref
+        print("model vs fixture: max rel %.3g, L2 rel %.3g"
+              % (np.abs(d).max() / np.abs(ref).max(), np.linalg.norm(d) / np.linalg.norm(ref)))
 
 
 if __name__ == "__main__":
