@@ -14,10 +14,11 @@ namespace dev {
 // after the inverse FFT, indexed [y][x] in FFT order.  With a.scratch set the
 // values that would be added at grid pixel (Y + y, X + x) are stored at
 // scratch[slot][y][x] instead (Y, X: the corner after the negative-w mirror),
-// for the ordered sum of gather_slots.
+// for the ordered sum of gather_slots.  ap: the [S][S] adder phasors
+// (a.adder_phasor or a copy of it in LDS).
 template <int S>
 __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
-                                      const float2* tile, float norm, int slot) {
+                                      const float2* tile, float norm, int slot, const float2* ap) {
     const int G = a.G;
     float2* const out = a.scratch ? a.scratch + (size_t)slot * S * S : nullptr;
     if (a.mode == P9IDG_GRID_PLAIN) {
@@ -25,7 +26,7 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
         for (int p = item_tid(); p < S * S; p += kThreads) {
             const int y = p / S, x = p - (p / S) * S;
             const int ys = (y + S / 2) % S, xs = (x + S / 2) % S;
-            float2 v = cmulf(tile[ys * S + xs], a.adder_phasor[p]);
+            float2 v = cmulf(tile[ys * S + xs], ap[p]);
             v.x *= norm;
             v.y *= norm;
             if (out) {
@@ -54,7 +55,7 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
         const int y_ = neg ? S - 1 - y : y;
         const int x_ = neg ? S - 1 - x : x;
         const int ys = (y_ + S / 2) % S, xs = (x_ + S / 2) % S;
-        float2 v = cmulf(a.adder_phasor[y_ * S + x_], tile[ys * S + xs]);
+        float2 v = cmulf(ap[y_ * S + x_], tile[ys * S + xs]);
         v.x *= norm;
         v.y *= norm;
         if (neg) v.y = -v.y;
@@ -66,6 +67,11 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
         unsafeAtomicAdd(&dst->x, v.x);
         unsafeAtomicAdd(&dst->y, v.y);
     }
+}
+template <int S>
+__device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
+                                      const float2* tile, float norm, int slot) {
+    adder<S>(a, g, tile, norm, slot, a.adder_phasor);
 }
 
 // Reads the subgrid from the model grid into `tile` (FFT order), zeros where

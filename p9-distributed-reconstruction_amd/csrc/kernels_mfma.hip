@@ -353,16 +353,30 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 // operand) within a tile, g picks the lane's 8 visibilities of the step.
 // Per step the x-phasors of the NT column tiles are built once, then each row
 // tile's V * y-phasor operands are built and multiplied into its NT tiles.
+// fresh: the plan's tables (rtab, twiddles, adder phasors) are not yet in
+// this workgroup's LDS (its first item, or an item of another plan).
 template <int S, class P>
-__device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, int k0, int k1, int slot) {
+__device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, int k0, int k1, int slot,
+                                                 bool fresh) {
     constexpr int NT = S / 16;  // tiles per axis
     static_assert(NT * 16 == S, "subgrid size a multiple of 16");
     constexpr int PER = 2;  // visibilities staged per thread and round
     constexpr int ROUND = PER * kThreads;
-    __shared__ double s_a[ROUND], s_b[ROUND], s_c[ROUND];
-    __shared__ float2 s_v[ROUND];
-    __shared__ float2 s_tile[2][S * S];
+    // s_buf holds a round's staged visibilities (s_a, s_b, s_c: ROUND doubles
+    // each; s_v: ROUND float2) followed by the two FFT tiles; after the last
+    // round the 4 waves' partial subgrids overlay it from the start, the last
+    // two in the FFT tiles.
+    constexpr int kStage = 4 * ROUND;  // float2 units
+    constexpr int kBuf = kStage + 2 * S * S > 4 * S * S ? kStage + 2 * S * S : 4 * S * S;
+    __shared__ float2 s_buf[kBuf];
+    __shared__ float s_rt[S * S];
+    __shared__ float2 s_ap[S * S], s_tw[S];
     __shared__ float s_red[4];
+    double* const s_a = reinterpret_cast<double*>(s_buf);
+    double* const s_b = s_a + ROUND;
+    double* const s_c = s_b + ROUND;
+    float2* const s_v = s_buf + 3 * ROUND;
+    float2* const tile0 = s_buf + (kBuf - 2 * S * S);
 
     const SubgridGeom g = load_geom(a, sidx, S);
     const SplitK sk = split_k(a);
@@ -372,8 +386,14 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
     const size_t vbase = ((size_t)g.bl * a.T + g.t0) * C;
     const bool chan_rec = a.chan_rec != 0;
 
-    for (int p = tid; p < S * S; p += kThreads) s_tile[0][p] = make_float2(0.f, 0.f);
-    __syncthreads();
+    if (fresh) {  // uniform over the workgroup
+        for (int p = tid; p < S * S; p += kThreads) {
+            s_rt[p] = a.rtab[p];
+            s_ap[p] = a.adder_phasor[p];
+        }
+        for (int p = tid; p < S; p += kThreads) s_tw[p] = a.twiddle[p];
+        __syncthreads();
+    }
 
     // FP16: scale the visibilities so |V| <= 1, and c * L by 2^-16.
     float vscale = 1.f;
@@ -510,33 +530,34 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
         __syncthreads();
     }
 
-    // subgrid = (M0 + 2 pi i r M1) / vscale, summed over the 4 waves into the
-    // ifftshift-ed tile (Gridder.cs:76-79; identity taper).  Lane (c, g) holds
-    // rows 4 g + e of every tile at column c.
+    // subgrid = (M0 + 2 pi i r M1) / vscale, ifftshift-ed (Gridder.cs:76-79;
+    // identity taper): each wave writes its part to its own tile, then the
+    // four are summed in wave order.  Lane (c, g) holds rows 4 g + e of every
+    // tile at column c.
     const float back = 1.f / vscale;
     const float back1 = back / cscale;
-    for (int w = 0; w < 4; w++) {
-        if (wave == w) {
+    float2* const part = s_buf + wave * S * S;
 #pragma unroll
-            for (int r = 0; r < NT; r++)
+    for (int r = 0; r < NT; r++)
 #pragma unroll
-                for (int q = 0; q < NT; q++)
+        for (int q = 0; q < NT; q++)
 #pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        const int y = 16 * r + 4 * grp + e, x = 16 * q + c;
-                        const float rr = kTwoPiF * a.rtab[y * S + x];
-                        const float m1r = acc1r[r][q][e] * back1, m1i = acc1i[r][q][e] * back1;
-                        const float gr = acc0r[r][q][e] * back - rr * m1i;
-                        const float gi = acc0i[r][q][e] * back + rr * m1r;
-                        float2& d = s_tile[0][((y + S / 2) % S) * S + (x + S / 2) % S];
-                        d.x += gr;
-                        d.y += gi;
-                    }
-        }
-        __syncthreads();
+            for (int e = 0; e < 4; e++) {
+                const int y = 16 * r + 4 * grp + e, x = 16 * q + c;
+                const float rr = kTwoPiF * s_rt[y * S + x];
+                const float m1r = acc1r[r][q][e] * back1, m1i = acc1i[r][q][e] * back1;
+                const float gr = acc0r[r][q][e] * back - rr * m1i;
+                const float gi = acc0i[r][q][e] * back + rr * m1r;
+                part[((y + S / 2) % S) * S + (x + S / 2) % S] = make_float2(gr, gi);
+            }
+    __syncthreads();
+    for (int p = tid; p < S * S; p += kThreads) {
+        const float2 p0 = s_buf[p], p1 = s_buf[S * S + p], p2 = s_buf[2 * S * S + p], p3 = s_buf[3 * S * S + p];
+        tile0[p] = make_float2(((p0.x + p1.x) + p2.x) + p3.x, ((p0.y + p1.y) + p2.y) + p3.y);
     }
-    const float2* res = fft_2d<S>(s_tile[0], s_tile[1], a.twiddle, +1);
-    adder<S>(a, g, res, 1.0f / (float)(S * S), slot);
+    __syncthreads();
+    const float2* res = fft_2d<S>(tile0, tile0 + S * S, s_tw, +1);
+    adder<S>(a, g, res, 1.0f / (float)(S * S), slot, s_ap);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,20 +1004,25 @@ template <int S>
 constexpr int kGridWaves = S <= 32 ? 2 : 1;
 // multiples of 16 up to 48 on 16x16x32 tiles (as many as the subgrid holds),
 // S = 8, 24 and 64 on 32x32x16
+// fresh: as for grid_mfma16_body
 template <int S, class P>
-__device__ __forceinline__ void grid_mfma_any(const KernelArgs& a, int s, int k0, int k1, int slot) {
+__device__ __forceinline__ void grid_mfma_any(const KernelArgs& a, int s, int k0, int k1, int slot, bool fresh) {
     if constexpr (S % 16 == 0 && S <= 48)
-        grid_mfma16_body<S, P>(a, s, k0, k1, slot);
+        grid_mfma16_body<S, P>(a, s, k0, k1, slot, fresh);
     else
         grid_mfma_body<S, P>(a, s, k0, k1, slot);
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_kernel(KernelArgs a) {
-    grid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1, a.order[blockIdx.x]);
+    grid_mfma_any<S, P>(a, a.order[blockIdx.x], 0, -1, a.order[blockIdx.x], true);
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_queue_kernel(KernelArgs a) {
-    run_queue(a, [&](int s, int k0, int k1, int slot) { grid_mfma_any<S, P>(a, s, k0, k1, slot); });
+    bool fresh = true;
+    run_queue(a, [&](int s, int k0, int k1, int slot) {
+        grid_mfma_any<S, P>(a, s, k0, k1, slot, fresh);
+        fresh = false;
+    });
 }
 
 // S <= 32: 16x16x32 tiles, two waves per SIMD; S = 48: 3 row tiles and two
@@ -1021,7 +1047,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
 
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_batch_kernel(BatchArgs b) {
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int slot) { grid_mfma_any<S, P>(a, s, k0, k1, slot); });
+    // the tables are per plan: reload them when the item's plan changes
+    const float* loaded = nullptr;
+    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int slot) {
+        grid_mfma_any<S, P>(a, s, k0, k1, slot, a.rtab != loaded);
+        loaded = a.rtab;
+    });
 }
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegridWaves<S>))) void degrid_mfma_batch_kernel(BatchArgs b) {
