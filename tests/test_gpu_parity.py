@@ -164,12 +164,12 @@ def test_single_channel_and_single_baseline(O, n_ant, C, kernel):
 
 
 REC_CASES = [
-    # subgrid, channels, frequency jitter (Hz), work-item size: with even
-    # spacing and C % 8 == 0 the gridder steps its phasors channel to
-    # channel, re-anchored every 8 visibilities and at every item start
-    # (items of 96 start mid-timestep for C = 24, 11, 14); the degridder
-    # steps them for any C with even spacing; uneven spacing (and, in the
-    # gridder, C % 8 != 0) takes the direct evaluation
+    # subgrid, channels, frequency jitter (Hz), work-item size (0: whole
+    # subgrids): with even spacing and C % 8 == 0 the gridder steps its
+    # phasors channel to channel, re-anchored every 8 visibilities and at
+    # every item start (items of 96 start mid-timestep for C = 24, 11, 14); the
+    # degridder steps them for any C with even spacing; uneven spacing (and,
+    # in the gridder, C % 8 != 0) takes the direct evaluation
     (16, 24, 0.0, 96),
     (32, 16, 0.0, 160),
     (32, 24, 0.0, 96),
@@ -183,6 +183,15 @@ REC_CASES = [
     (48, 8, 3.0e5, 64),
     (48, 14, 0.0, 96),
     (48, 9, 0.0, 160),
+    (16, 9, 0.0, 0),
+    (32, 11, 0.0, 0),
+    (32, 13, 0.0, 0),
+    (32, 17, 0.0, 0),
+    (32, 100, 0.0, 0),
+    (32, 10, 3.0e5, 0),
+    (48, 9, 0.0, 0),
+    (48, 15, 0.0, 0),
+    (24, 11, 0.0, 0),
 ]
 
 
