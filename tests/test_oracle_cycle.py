@@ -181,3 +181,50 @@ def test_reconstruct_recovers_a_point_source(O):
     assert np.all(x >= 0)
     y, xx = np.unravel_index(np.argmax(x), x.shape)
     assert (y, xx) == (32 - 4, 32 + 6)
+
+
+@pytest.mark.parametrize("lam", [1.0, 200.0])
+def test_reconstruct_lambda_rule(O, lam):
+    """Every cycle's lambda follows MajorCycle.cs:98-100, recomputed here from
+    the cycle's own max dirty / max gradient and the PSF's cut, sidelobe and
+    Lipschitz figures taken with numpy slices: lambda = max(maxB * sidelobe *
+    max(maxB / (maxDirty * lip), 1) / alpha, lam * lip) (float32 products and
+    casts as in the C#).  On this sky the correction factor is ~1.12, so a
+    dropped correction changes lambda by 12%; lam = 200 puts every cycle on
+    the Lipschitz floor."""
+    cell = obs.ARCSEC * 30.0
+    rng = np.random.default_rng(91)
+    src = [(1.0, 6 * cell, -4 * cell, 0.0)] + [
+        (float(rng.uniform(0.3, 0.8)), float(rng.integers(-3, 4) + 2) * cell, float(rng.integers(-3, 4)) * cell, 0.0)
+        for _ in range(8)]
+    ob = obs.custom(G=64, S=16, K=4, L=1, n_ant=8, T=24, C=4, seed=91, core_diam=600.0, cell_arcsec=30.0,
+                    sources=src, noise=0.01, wstep=0.0)
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    m = part.subgrids
+    alpha, factor = 0.01, 4
+    x, stats = O.reconstruct_serial_cd(oc, m, ob.vis, None, ob.uvw, ob.freq, factor, 4, lam, alpha, 500, 1e-5)
+    # lam = 200: the first cycle's proximal steps are all zero and the cycle
+    # loop stops there (abs max dx < MAJOR_EPSILON, MajorCycle.cs:104,122-123)
+    assert len(stats) == (4 if lam == 1.0 else 1)
+    psf = O.gridw_to_image(oc, m, np.ones_like(ob.vis), ob.uvw, ob.freq)
+    G = psf.shape[0]
+    pc = G // factor
+    o = G // 2 - pc // 2
+    cut = psf[o:o + pc, o:o + pc]
+    outside = np.abs(psf).copy()
+    outside[o:o + pc, o:o + pc] = 0
+    sidelobe = np.float32(outside.max())
+    lip = float(np.sum((cut * cut).astype(np.float64)))
+    floor = float(np.float32(float(np.float32(lam)) * lip))
+    f32 = np.float32
+    corrections = []
+    for s in stats:
+        corr = max(float(s["max_gradient"]) / (float(s["max_dirty"]) * lip), 1.0)
+        corrections.append(corr)
+        expect = f32(max(float(f32(s["max_gradient"]) * sidelobe) * corr / float(f32(alpha)), floor))
+        assert s["lambda_"] == expect
+    if lam == 1.0:
+        assert min(corrections) > 1.05 and all(s["lambda_"] > floor for s in stats)
+    else:
+        assert all(s["lambda_"] == f32(floor) for s in stats)
