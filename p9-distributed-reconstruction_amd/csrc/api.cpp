@@ -819,8 +819,7 @@ int p9idg_partition(const p9idg_constants* c, const double* uvw, int32_t nr_base
 int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
     if (!c || !plan) return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
     *plan = nullptr;
-    if (c->grid_size <= 0 || (c->grid_size & 1))
-        return fail(P9IDG_ERR_UNSUPPORTED, "grid_size must be positive and even");
+    if (c->grid_size <= 0) return fail(P9IDG_ERR_ARGUMENT, "grid_size must be positive");
     if (!subgrid_size_supported(c->subgrid_size))
         return fail(P9IDG_ERR_UNSUPPORTED, "subgrid_size must be one of 8, 16, 24, 32, 48, 64");
     if (c->subgrid_size >= c->grid_size) return fail(P9IDG_ERR_ARGUMENT, "subgrid_size >= grid_size");

@@ -33,7 +33,7 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
                 out[p] = v;
                 continue;
             }
-            float2* dst = a.grid + (size_t)shifted(g.V + y, G) * G + shifted(g.U + x, G);
+            float2* dst = a.grid + shift_index(g.V + y, g.U + x, G);
             unsafeAtomicAdd(&dst->x, v.x);
             unsafeAtomicAdd(&dst->y, v.y);
         }
@@ -63,7 +63,7 @@ __device__ __forceinline__ void adder(const KernelArgs& a, const SubgridGeom& g,
             out[p] = v;
             continue;
         }
-        float2* dst = layer + (size_t)shifted(Y + y, G) * G + shifted(X + x, G);
+        float2* dst = layer + shift_index(Y + y, X + x, G);
         unsafeAtomicAdd(&dst->x, v.x);
         unsafeAtomicAdd(&dst->y, v.y);
     }
@@ -88,7 +88,7 @@ __device__ __forceinline__ void splitter(const KernelArgs& a, const SubgridGeom&
             const int yd = (y + S / 2) % S, xd = (x + S / 2) % S;
             float2 v = make_float2(0.f, 0.f);
             if (ok) {
-                float2 gv = a.model_grid[(size_t)shifted(g.V + y, G) * G + shifted(g.U + x, G)];
+                float2 gv = a.model_grid[shift_index(g.V + y, g.U + x, G)];
                 v = cmulf(gv, conjf2(a.adder_phasor[p]));
             }
             tile[yd * S + xd] = v;
@@ -107,7 +107,7 @@ __device__ __forceinline__ void splitter(const KernelArgs& a, const SubgridGeom&
         if (ok) {
             const int xs = neg ? G - X - x : X + x;
             const int ys = neg ? G - Y - y : Y + y;
-            float2 gv = a.model_grid[(size_t)shifted(ys, G) * G + shifted(xs, G)];
+            float2 gv = a.model_grid[shift_index(ys, xs, G)];
             if (neg) gv.y = -gv.y;
             v = cmulf(conjf2(a.adder_phasor[p]), gv);
         }

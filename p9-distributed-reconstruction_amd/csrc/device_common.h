@@ -246,10 +246,13 @@ __device__ __forceinline__ void run_batch_queue(const BatchArgs& b, Body body) {
     }
 }
 
-// Index of grid row/column r after FFT.Shift (quadrant swap) for even G.
-__device__ __forceinline__ int shifted(int r, int G) {
+// Linear index of grid element (y, x) after FFT.Shift (FFT.cs:248-264): the
+// h x h blocks (h = G / 2) swap 1 <-> 3 and 2 <-> 4; for odd G the last row
+// and the last column stay in place.  The map is its own inverse.
+__device__ __forceinline__ size_t shift_index(int y, int x, int G) {
     const int h = G / 2;
-    return r >= h ? r - h : r + h;
+    if ((G & 1) && (y == 2 * h || x == 2 * h)) return (size_t)y * G + x;
+    return (size_t)(y >= h ? y - h : y + h) * G + (x >= h ? x - h : x + h);
 }
 
 }  // namespace dev

@@ -41,7 +41,7 @@ __global__ void layers_to_image_kernel(const float2* layers, int L, int G, doubl
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < GG; i += stride) {
         const int y = (int)(i / G), x = (int)(i - (size_t)y * G);
-        const size_t src = shift ? (size_t)shifted(y, G) * G + shifted(x, G) : i;
+        const size_t src = shift ? shift_index(y, x, G) : i;
         float acc = 0.f;
         for (int k = 0; k < L; k++) acc += (float)((double)layers[k * GG + src].x * inv_n);
         img[i] = acc;
@@ -75,7 +75,7 @@ __global__ void sum_layer_boxes_kernel(const float2* layers, const Box* boxes, i
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
         const int y = u.y0 + (int)(i / w), x = u.x0 + (int)(i % w);
-        const size_t idx = (size_t)shifted(y, G) * G + shifted(x, G);
+        const size_t idx = shift_index(y, x, G);
         float2 acc = make_float2(0.f, 0.f);
         for (int b = 0; b < nboxes; b++) {
             const Box bx = boxes[b];
@@ -97,7 +97,7 @@ __global__ void grid_to_image_f64_kernel(const float2* grid, int G, double inv_n
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < GG; i += stride) {
         const int y = (int)(i / G), x = (int)(i - (size_t)y * G);
-        const size_t src = shift ? (size_t)shifted(y, G) * G + shifted(x, G) : i;
+        const size_t src = shift ? shift_index(y, x, G) : i;
         img[i] = (double)grid[src].x * inv_n;
     }
 }
@@ -110,7 +110,7 @@ __global__ void image_to_grid_kernel(const float* img, int G, int shift, float2*
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < GG; i += stride) {
         const int y = (int)(i / G), x = (int)(i - (size_t)y * G);
-        const size_t src = shift ? (size_t)shifted(y, G) * G + shifted(x, G) : i;
+        const size_t src = shift ? shift_index(y, x, G) : i;
         grid[i] = make_float2(img[src], 0.f);
     }
 }
@@ -143,7 +143,7 @@ __global__ void zero_boxes_kernel(float2* base, const Box* boxes, int G) {
     float2* layer = base + (size_t)b.layer * G * G;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int y = b.y0 + (int)(i / w), x = b.x0 + (int)(i % w);
-        layer[(size_t)shifted(y, G) * G + shifted(x, G)] = make_float2(0.f, 0.f);
+        layer[shift_index(y, x, G)] = make_float2(0.f, 0.f);
     }
 }
 
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void gather_bins_kernel(const GatherBin* bins,
             acc.x += v.x;
             acc.y += v.y;
         }
-        layer[(size_t)shifted(gy, G) * G + shifted(gx, G)] = acc;
+        layer[shift_index(gy, gx, G)] = acc;
     }
 }
 

@@ -89,7 +89,7 @@ extern "C" int p9idg_grid_noalloc(const p9idg_constants* c, int32_t kernel, int3
     if (!c || !freq || !d_uvw || !d_vis || !d_out || (!meta && nsub > 0))
         return fail(P9IDG_ERR_ARGUMENT, "NULL argument");
     const int G = c->grid_size, S = c->subgrid_size, L = c->wlayer_count;
-    if (G <= 0 || (G & 1)) return fail(P9IDG_ERR_UNSUPPORTED, "grid_size must be positive and even");
+    if (G <= 0) return fail(P9IDG_ERR_ARGUMENT, "grid_size must be positive");
     if (!subgrid_size_supported(S)) return fail(P9IDG_ERR_UNSUPPORTED, "subgrid_size must be one of 8, 16, 24, 32, 48, 64");
     if (S >= G) return fail(P9IDG_ERR_ARGUMENT, "subgrid_size >= grid_size");
     if (L < 1) return fail(P9IDG_ERR_ARGUMENT, "wlayer_count must be >= 1");

@@ -146,6 +146,50 @@ def test_grid_size_not_a_multiple_of_32(O, G, S, kernel):
         assert np.array_equal(plan.gridw(ob.vis), det)
 
 
+@pytest.mark.parametrize("G,S", [(255, 32), (201, 24)])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_odd_grid_size(O, G, S, kernel):
+    """An odd grid size.  FFT.Shift (FFT.cs:248-264) swaps the G/2 x G/2
+    blocks and leaves the last row and column where they are; the oracle
+    follows it (test_oracle_fft.py).  Grid / DeGrid / ToImage / CalculatePSF /
+    ImageToVis on one layer, GridW / DeGridW on 4 w-layers with negative w,
+    atomic and ordered sums, against the oracle."""
+    ob = obs.custom(G=G, S=S, K=8, L=1, n_ant=10, T=40, C=6, seed=G + S, core_diam=1200.0, wstep=0.0)
+    part = ob.partition()
+    oc = oracle_constants(ob.c)
+    m = part.subgrids
+    ref_grid = O.grid(oc, m, ob.vis, ob.uvw, ob.freq)
+    ref_img = O.to_image(oc, m, ob.vis, ob.uvw, ob.freq)
+    ones = np.ones_like(ob.vis)
+    with p9idg.Plan(ob.c, kernel=kernel) as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        errs = {
+            "grid": rel_err(plan.grid(ob.vis), ref_grid),
+            "degrid": rel_err(plan.degrid(ref_grid), O.degrid(oc, m, ref_grid, ob.uvw, ob.freq)),
+            "to_image": rel_err(plan.to_image(ob.vis), ref_img),
+            "psf": rel_err(plan.calculate_psf(None), O.to_image(oc, m, ones, ob.uvw, ob.freq)),
+            "image_to_vis": rel_err(plan.image_to_vis(ref_img),
+                                    O.image_to_vis(oc, m, ref_img, ob.uvw, ob.freq)),
+        }
+    obw = obs.custom(G=G, S=S, K=8, L=4, n_ant=10, T=40, C=6, seed=G + S + 1, core_diam=1500.0)
+    partw = obw.partition()
+    ocw = oracle_constants(obw.c)
+    mw = partw.subgrids
+    assert (mw["WLambda"] < 0).any()
+    ref_l = O.gridw(ocw, mw, obw.vis, obw.uvw, obw.freq)
+    ref_v = O.degridw(ocw, mw, ref_l.sum(axis=0), obw.uvw, obw.freq)
+    with p9idg.Plan(obw.c, kernel=kernel) as plan:
+        plan.set_observation(obw.uvw, obw.freq, partw)
+        errs["gridw"] = rel_err(plan.gridw(obw.vis), ref_l)
+        errs["degridw"] = rel_err(plan.degridw(ref_l.sum(axis=0)), ref_v)
+        plan.set_deterministic(True)
+        det = plan.gridw(obw.vis)
+        errs["gridw_ordered"] = rel_err(det, ref_l)
+        assert np.array_equal(plan.gridw(obw.vis), det)
+    print(kernel, G, S, {k: "%.2e" % v for k, v in errs.items()})
+    assert max(errs.values()) <= TOL, errs
+
+
 @pytest.mark.parametrize("n_ant,C", [(10, 1), (2, 8), (2, 1)])
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_single_channel_and_single_baseline(O, n_ant, C, kernel):
