@@ -696,6 +696,12 @@ def main():
         "flops_per_launch": flops_per_launch,
         "kernel_ms": dom_ms,
     }
+    if args.kernel.startswith("mfma") and S in (16, 32, 48):
+        # the split product issues 16 16x16x32 MFMAs (three hi/lo terms of four
+        # real products for M0, four hi-only ones for M1) per 16 x 16 tile and
+        # 32 visibilities, where the complex multiply-add needs the work of 4
+        roofline["mfma_issued_per_algorithmic"] = 4.0
+        roofline["issued_frac"] = 4.0 * achieved / peak
     cpu, err = None, None
     if not args.no_oracle:
         cpu, err = oracle_checks(ob, part, gpu, model_grid)
