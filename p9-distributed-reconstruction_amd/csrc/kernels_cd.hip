@@ -20,6 +20,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kSelectThreads = 1024;
+static_assert(kTile * kTile % kThreads == 0, "whole pixels per thread");
 
 // Common.cs:27-29; Math.Max(v, 0f) gives +0 for -0
 __device__ __forceinline__ float prox(float x, float lipschitz, float lambda, float alpha) {
@@ -198,12 +199,49 @@ __device__ Cand wave_best(const Cand* v, int n, int lane) {
 
 // Rows [ty0, ty1] of the row table from the tile table, then the overall best
 // (4 waves), and the selection of FastSerialCD.cs:84-97 by thread 0.
-__device__ void rows_and_select(const Cand* tiles, Cand* rows, int ntx, int nty, int ty0, int ty1, State* st,
-                                float* x) {
+// A wave takes rows ty0 + wave, + 4, + 8, ...; four of its rows at a time have
+// their tiles loaded before any is reduced, so the loads overlap (the
+// comparisons run in the same order as one row at a time).
+__device__ __forceinline__ void rows_and_select(const Cand* __restrict__ tiles, Cand* __restrict__ rows, int ntx, int nty, int ty0,
+                                int ty1, State* st, float* x) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int ty = ty0 + wave; ty <= ty1; ty += kThreads / 64) {
-        const Cand b = wave_best(tiles + (size_t)ty * ntx, ntx, lane);
-        if (lane == 0) rows[ty] = b;
+    constexpr int kWaves = kThreads / 64, kBatch = 4;
+    for (int ty = ty0 + wave; ty <= ty1; ty += kWaves * kBatch) {
+        Cand b[kBatch];
+        if (ntx <= 128) {  // two tiles per lane at most: all loads first
+            Cand v0[kBatch], v1[kBatch];
+#pragma unroll
+            for (int r = 0; r < kBatch; r++) {
+                const int t = ty + r * kWaves;
+                const Cand* row = tiles + (size_t)t * ntx;
+                v0[r] = t <= ty1 && lane < ntx ? row[lane] : none();
+                v1[r] = t <= ty1 && lane + 64 < ntx ? row[lane + 64] : none();
+            }
+#pragma unroll
+            for (int r = 0; r < kBatch; r++) {
+                b[r] = none();
+                if (better(v0[r], b[r])) b[r] = v0[r];
+                if (better(v1[r], b[r])) b[r] = v1[r];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < kBatch; r++) {
+                const int t = ty + r * kWaves;
+                b[r] = none();
+                if (t <= ty1)
+                    for (int i = lane; i < ntx; i += 64)
+                        if (better(tiles[(size_t)t * ntx + i], b[r])) b[r] = tiles[(size_t)t * ntx + i];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kBatch; r++) {
+            for (int m = 32; m > 0; m >>= 1) {
+                const Cand o = shfl_xor(b[r], m);
+                if (better(o, b[r])) b[r] = o;
+            }
+            const int t = ty + r * kWaves;
+            if (lane == 0 && t <= ty1) rows[t] = b[r];
+        }
     }
     __threadfence_block();
     __syncthreads();
@@ -237,9 +275,10 @@ __global__ __launch_bounds__(kThreads) void first_select_kernel(const Cand* tile
 // (FastSerialCD.cs:219-238) and re-evaluates the tile.  The change of the
 // last selection is applied even when that selection reached the iteration
 // limit (the reference updates before it stops).
-__global__ __launch_bounds__(kThreads) void window_kernel(float* x, float* g, const float* amap, const float* psf2,
-                                                          int p2h, int p2w, int H, int W, Cand* tiles,
-                                                          const State* st) {
+__global__ __launch_bounds__(kThreads) void window_kernel(float* __restrict__ x, float* __restrict__ g,
+                                                          const float* __restrict__ amap,
+                                                          const float* __restrict__ psf2, int p2h, int p2w, int H,
+                                                          int W, Cand* __restrict__ tiles, const State* st) {
     __shared__ Cand sh[kThreads / 64];
     if (!st->updated) return;  // uniform
     const float lambda = st->lambda, alpha = st->alpha;
@@ -251,16 +290,37 @@ __global__ __launch_bounds__(kThreads) void window_kernel(float* x, float* g, co
     const int c0 = max(x0 - xh, 0), c1 = min(x0 - xh + p2w, W);
     const int ty = r0 / kTile + (int)blockIdx.y, tx = c0 / kTile + (int)blockIdx.x;
     if (ty > (r1 - 1) / kTile || tx > (c1 - 1) / kTile) return;
-    Cand best = none();
-    for (int p = threadIdx.x; p < kTile * kTile; p += kThreads) {
+    // this thread's pixels: every load first, then the updates and
+    // evaluations in pixel order (x is written only by the select kernel,
+    // each gradient by its own thread)
+    constexpr int kPer = kTile * kTile / kThreads;
+    int idx[kPer];
+    bool in[kPer], win[kPer];
+    float gv[kPer], pv[kPer], xv[kPer], av[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int p = (int)threadIdx.x + k * kThreads;
         const int i = ty * kTile + p / kTile, j = tx * kTile + p % kTile;
-        if (i < H && j < W) {
-            const int idx = i * W + j;
-            if (i >= r0 && i < r1 && j >= c0 && j < c1)
-                g[idx] = __fadd_rn(g[idx], __fmul_rn(psf2[(i - y0 + yh) * p2w + (j - x0 + xh)], d));
-            const Cand c = eval(x, g, amap, idx, lambda, alpha);
-            if (better(c, best)) best = c;
+        in[k] = i < H && j < W;
+        win[k] = in[k] && i >= r0 && i < r1 && j >= c0 && j < c1;
+        idx[k] = in[k] ? i * W + j : 0;
+        gv[k] = in[k] ? g[idx[k]] : 0.f;
+        xv[k] = in[k] ? x[idx[k]] : 0.f;
+        av[k] = in[k] ? amap[idx[k]] : 0.f;
+        pv[k] = win[k] ? psf2[(i - y0 + yh) * p2w + (j - x0 + xh)] : 0.f;
+    }
+    Cand best = none();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        if (!in[k]) continue;
+        if (win[k]) {
+            gv[k] = __fadd_rn(gv[k], __fmul_rn(pv[k], d));
+            g[idx[k]] = gv[k];
         }
+        // FastSerialCD.cs:150-162 (as eval)
+        const float t = prox(__fadd_rn(__fmul_rn(xv[k], av[k]), gv[k]), av[k], lambda, alpha);
+        const Cand c{fabsf(__fsub_rn(xv[k], t)), idx[k], t, xv[k]};
+        if (better(c, best)) best = c;
     }
     best = block_best<kThreads>(best, sh);
     if (threadIdx.x == 0) tiles[ty * ntx + tx] = best;
