@@ -36,21 +36,48 @@ __device__ __forceinline__ bool better(const Cand& a, const Cand& b) {
     return a.diff > b.diff || (a.diff == b.diff && a.diff > 0.f && a.idx < b.idx);
 }
 
-__device__ __forceinline__ Cand shfl_xor(const Cand& c, int m) {
-    Cand o;
-    o.diff = __shfl_xor(c.diff, m);
-    o.idx = __shfl_xor(c.idx, m);
-    o.xnew = __shfl_xor(c.xnew, m);
-    o.xold = __shfl_xor(c.xold, m);
-    return o;
-}
 
+// One step of a wave-wide best over DPP lane routes: each lane compares with
+// the candidate CTRL routes to it; lanes outside ROW_MASK (rows of 16 lanes)
+// keep their own.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ Cand dpp_best(const Cand& c) {
+    Cand o;
+    o.diff = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, c.diff),
+                                                                   __builtin_bit_cast(int, c.diff), CTRL, ROW_MASK,
+                                                                   0xF, false));
+    o.idx = __builtin_amdgcn_update_dpp(c.idx, c.idx, CTRL, ROW_MASK, 0xF, false);
+    o.xnew = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, c.xnew),
+                                                                   __builtin_bit_cast(int, c.xnew), CTRL, ROW_MASK,
+                                                                   0xF, false));
+    o.xold = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, c.xold),
+                                                                   __builtin_bit_cast(int, c.xold), CTRL, ROW_MASK,
+                                                                   0xF, false));
+    return better(o, c) ? o : c;
+}
+// The wave's best candidate, in every lane: lane pairs, quads (quad
+// permutes), 8 and 16 lanes (half-row and row mirrors), then rows 0 + 1 into
+// row 1 and rows 2 + 3 into row 3 (row_bcast15), and rows 0-1 into rows 2-3
+// (row_bcast31), so lane 63 holds the best of all 64.  better() orders every
+// candidate with a positive difference strictly, so that best does not depend
+// on the order of the comparisons.
+__device__ __forceinline__ Cand wave_reduce(Cand c) {
+    c = dpp_best<0xB1, 0xF>(c);   // quad_perm [1, 0, 3, 2]
+    c = dpp_best<0x4E, 0xF>(c);   // quad_perm [2, 3, 0, 1]
+    c = dpp_best<0x141, 0xF>(c);  // row_half_mirror
+    c = dpp_best<0x140, 0xF>(c);  // row_mirror
+    c = dpp_best<0x142, 0xA>(c);  // row_bcast15
+    c = dpp_best<0x143, 0xC>(c);  // row_bcast31
+    Cand r;
+    r.diff = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.diff), 63));
+    r.idx = __builtin_amdgcn_readlane(c.idx, 63);
+    r.xnew = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.xnew), 63));
+    r.xold = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.xold), 63));
+    return r;
+}
 template <int NT>
 __device__ Cand block_best(Cand c, Cand* sh) {
-    for (int m = 32; m > 0; m >>= 1) {
-        const Cand o = shfl_xor(c, m);
-        if (better(o, c)) c = o;
-    }
+    c = wave_reduce(c);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) sh[wave] = c;
     __syncthreads();
@@ -190,11 +217,7 @@ __device__ Cand wave_best(const Cand* v, int n, int lane) {
     Cand b = none();
     for (int i = lane; i < n; i += 64)
         if (better(v[i], b)) b = v[i];
-    for (int m = 32; m > 0; m >>= 1) {
-        const Cand o = shfl_xor(b, m);
-        if (better(o, b)) b = o;
-    }
-    return b;
+    return wave_reduce(b);
 }
 
 // Rows [ty0, ty1] of the row table from the tile table, then the overall best
@@ -235,10 +258,7 @@ __device__ __forceinline__ void rows_and_select(const Cand* __restrict__ tiles, 
         }
 #pragma unroll
         for (int r = 0; r < kBatch; r++) {
-            for (int m = 32; m > 0; m >>= 1) {
-                const Cand o = shfl_xor(b[r], m);
-                if (better(o, b[r])) b[r] = o;
-            }
+            b[r] = wave_reduce(b[r]);
             const int t = ty + r * kWaves;
             if (lane == 0 && t <= ty1) rows[t] = b[r];
         }
