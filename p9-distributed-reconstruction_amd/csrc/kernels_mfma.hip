@@ -157,6 +157,20 @@ __device__ __forceinline__ float block_max(float v, float* red) {
 constexpr double kInvTwoPi = 1.0 / kTwoPi;
 constexpr float kTwoPiF = 6.2831853071795864769f;
 
+// x[l] + x[l ^ 16] (rows of 16 lanes) and x[l] + x[l ^ 32] (half waves) in
+// every lane l, over the row and half-wave swaps: VALU instructions, so the
+// sums wait on no LDS round trip (and no LDS counter shared with the tile
+// loads).  Each swap leaves x[l] in one output and x[l ^ 16] (x[l ^ 32]) in
+// the other, and the sum of the two is the same in either order.
+__device__ __forceinline__ float sum_xor16(float x) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float sum_xor32(float x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 // ---------------------------------------------------------------------------
 // Gridder on 32x32x16 MFMA tiles, used for S = 8, 24 and 64.  One 256-thread
 // workgroup per subgrid.  For S <= 32 the subgrid is one 32 x 32 MFMA tile
@@ -725,10 +739,8 @@ __device__ __forceinline__ void degrid_mfma_body(const KernelArgs& a, int sidx, 
                 si = fmaf(b.x, zi, fmaf(-b.y, zr, si));
             }
         }
-        sr += __shfl_xor(sr, 16);
-        si += __shfl_xor(si, 16);
-        sr += __shfl_xor(sr, 32);
-        si += __shfl_xor(si, 32);
+        sr = sum_xor32(sum_xor16(sr));
+        si = sum_xor32(sum_xor16(si));
         return make_float2(sr * out_scale, si * out_scale);
     };
     // x of phasor e and y of phasor i for this lane; phasors outside the
@@ -993,8 +1005,8 @@ __device__ __forceinline__ void degrid_mfma_wide_body(const KernelArgs& a, int s
                 }
             }
         }
-        sr += __shfl_xor(sr, 32);
-        si += __shfl_xor(si, 32);
+        sr = sum_xor32(sr);
+        si = sum_xor32(si);
         if (grp == 0 && j < K) a.vis_out[vbase + j] = make_float2(sr * out_scale, si * out_scale);
     }
 }
