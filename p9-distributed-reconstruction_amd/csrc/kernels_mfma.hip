@@ -382,7 +382,11 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
     // two in the FFT tiles.
     constexpr int kStage = 4 * ROUND;  // float2 units
     constexpr int kBuf = kStage + 2 * S * S > 4 * S * S ? kStage + 2 * S * S : 4 * S * S;
+    // the in-place sum of the partials into the first FFT tile needs that
+    // tile to be partial 2 exactly, or clear of all four
+    static_assert(kBuf - 2 * S * S == 2 * S * S || kBuf - 2 * S * S >= 4 * S * S, "partials vs FFT tiles");
     __shared__ float2 s_buf[kBuf];
+    __shared__ float s_cf[ROUND];  // the staged c as FP32 (scaled for FP16)
     __shared__ float s_rt[S * S];
     __shared__ float2 s_ap[S * S], s_tw[S];
     __shared__ float s_red[4];
@@ -475,9 +479,11 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
         for (int i = 0; i < PER; i++) {
             const int idx = i * kThreads + tid;
             const bool in = j0 + idx < K;
+            const double cc = in ? (g.w_off - fk[i] * fw[i]) * kInvTwoPi : 0.0;
             s_a[idx] = in ? (g.u_off - fk[i] * fu[i]) * kInvTwoPi : 0.0;
             s_b[idx] = in ? (g.v_off - fk[i] * fv[i]) * kInvTwoPi : 0.0;
-            s_c[idx] = in ? (g.w_off - fk[i] * fw[i]) * kInvTwoPi : 0.0;
+            s_c[idx] = cc;
+            s_cf[idx] = (float)cc * cscale;
             s_v[idx] = in ? make_float2(fvis[i].x * vscale, fvis[i].y * vscale) : make_float2(0.f, 0.f);
         }
         __syncthreads();
@@ -503,7 +509,7 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
             // c of the 8 visibilities (FP32; scaled for FP16)
             float cf[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) cf[e] = (float)s_c[jg + e] * cscale;
+            for (int e = 0; e < 8; e++) cf[e] = s_cf[jg + e];
 #pragma unroll
             for (int r = 0; r < NT; r++) {
                 float2 B[8];
