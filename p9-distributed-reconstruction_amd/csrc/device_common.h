@@ -216,9 +216,27 @@ __device__ __forceinline__ float2* fft_2d(float2* a, float2* b, const float2* tw
 // Persistent work queue: each workgroup takes the next WorkItem from the
 // counter until every item is taken; body(sidx, k0, k1) processes one.  All
 // threads of the workgroup leave together once the counter passes nitems.
-template <class Body>
+// AHEAD: the next item is claimed as soon as the current one is known, so the
+// counter's round trip overlaps the current item's work; the workgroup then
+// holds one claimed item while it works, which can lengthen the tail (on the
+// GPU the MFMA gridder gains from it and the degridder does not).
+template <bool AHEAD = false, class Body>
 __device__ __forceinline__ void run_queue(const KernelArgs& a, Body body) {
     __shared__ int s_item;
+    if (AHEAD) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+        __syncthreads();
+        int it = s_item;
+        while (it < a.nitems) {  // uniform
+            const WorkItem w = a.items[it];
+            __syncthreads();  // every thread has read s_item
+            if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+            body(w.sidx, w.k0, w.k1, w.slot);
+            __syncthreads();  // this item's LDS reads are done; s_item is the next
+            it = s_item;
+        }
+        return;
+    }
     for (;;) {
         __syncthreads();  // previous item's LDS reads are done
         if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
@@ -231,9 +249,24 @@ __device__ __forceinline__ void run_queue(const KernelArgs& a, Body body) {
 }
 
 // Batched queue: like run_queue, with each item's observation in args[inst].
-template <class Body>
+template <bool AHEAD = false, class Body>
 __device__ __forceinline__ void run_batch_queue(const BatchArgs& b, Body body) {
     __shared__ int s_item;
+    if (AHEAD) {
+        if (threadIdx.x == 0) s_item = atomicAdd(b.counter, 1);
+        __syncthreads();
+        int it = s_item;
+        while (it < b.nitems) {  // uniform
+            const WorkItem w = b.items[it];
+            __syncthreads();  // every thread has read s_item
+            if (threadIdx.x == 0) s_item = atomicAdd(b.counter, 1);
+            const KernelArgs a = b.args[w.inst];
+            body(a, w.sidx, w.k0, w.k1, w.slot);
+            __syncthreads();  // this item's LDS reads are done; s_item is the next
+            it = s_item;
+        }
+        return;
+    }
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(b.counter, 1);

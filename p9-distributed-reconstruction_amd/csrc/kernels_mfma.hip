@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridW
 template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_queue_kernel(KernelArgs a) {
     bool fresh = true;
-    run_queue(a, [&](int s, int k0, int k1, int slot) {
+    run_queue<true>(a, [&](int s, int k0, int k1, int slot) {
         grid_mfma_any<S, P>(a, s, k0, k1, slot, fresh);
         fresh = false;
     });
@@ -1067,7 +1067,7 @@ template <int S, class P>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kGridWaves<S>))) void grid_mfma_batch_kernel(BatchArgs b) {
     // the tables are per plan: reload them when the item's plan changes
     const float* loaded = nullptr;
-    run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int slot) {
+    run_batch_queue<true>(b, [&](const KernelArgs& a, int s, int k0, int k1, int slot) {
         grid_mfma_any<S, P>(a, s, k0, k1, slot, a.rtab != loaded);
         loaded = a.rtab;
     });
