@@ -439,12 +439,25 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
 #pragma unroll
         for (int q = 0; q < NT; q++) acc0r[r][q] = acc0i[r][q] = acc1r[r][q] = acc1i[r][q] = (f32x4){};
 
+    // Each wave takes a contiguous quarter of the item's steps (32 visibilities
+    // each) and stages them WROUND at a time in its own quarter of the staging
+    // arrays: a wave's LDS operations run in order, so the waves need no
+    // workgroup barrier until their partial tiles.
+    constexpr int WROUND = ROUND / 4;
+    const int nst = (K - k0 + 31) / 32;
+    const int jw0 = k0 + 32 * (nst * wave / 4), jw1 = min(K, k0 + 32 * (nst * (wave + 1) / 4));
+    double* const w_a = s_a + wave * WROUND;
+    double* const w_b = s_b + wave * WROUND;
+    double* const w_c = s_c + wave * WROUND;
+    float* const w_cf = s_cf + wave * WROUND;
+    float2* const w_v = s_v + wave * WROUND;
+
     double fu[PER], fv[PER], fw[PER], fk[PER];
     float2 fvis[PER];
     auto fetch = [&](int j0) {
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int jc = min(j0 + i * kThreads + tid, K - 1);
+            const int jc = min(j0 + i * 64 + lane, K - 1);
             const int t = g.t0 + jc / C, ch = jc % C;
             const double* p = a.uvw + ((size_t)g.bl * a.T + t) * 3;
             fu[i] = p[0];
@@ -458,8 +471,8 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
     // (ca, cc) for coordinate tile t: channel stepping or direct evaluation
     auto phasors = [&](int jg, const double* co, int t, float2 (&out)[8]) {
         if (chan_rec) {
-            const double ca = co[jg], cc = s_c[jg];
-            const double da = co[jg + 1] - ca, dc = s_c[jg + 1] - cc;
+            const double ca = co[jg], cc = w_c[jg];
+            const double da = co[jg + 1] - ca, dc = w_c[jg + 1] - cc;
             float2 A = cis_f64(fma(ca, lt[t], cc * nt[t]));
             const float2 R = cis_f64(fma(da, lt[t], dc * nt[t]));
 #pragma unroll
@@ -469,27 +482,27 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; e++) out[e] = cis_f64(fma(co[jg + e], lt[t], s_c[jg + e] * nt[t]));
+            for (int e = 0; e < 8; e++) out[e] = cis_f64(fma(co[jg + e], lt[t], w_c[jg + e] * nt[t]));
         }
     };
 
-    if (k0 < K) fetch(k0);
-    for (int j0 = k0; j0 < K; j0 += ROUND) {
+    if (jw0 < jw1) fetch(jw0);
+    for (int j0 = jw0; j0 < jw1; j0 += WROUND) {  // uniform over the wave
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int idx = i * kThreads + tid;
-            const bool in = j0 + idx < K;
+            const int idx = i * 64 + lane;
+            const bool in = j0 + idx < jw1;
             const double cc = in ? (g.w_off - fk[i] * fw[i]) * kInvTwoPi : 0.0;
-            s_a[idx] = in ? (g.u_off - fk[i] * fu[i]) * kInvTwoPi : 0.0;
-            s_b[idx] = in ? (g.v_off - fk[i] * fv[i]) * kInvTwoPi : 0.0;
-            s_c[idx] = cc;
-            s_cf[idx] = (float)cc * cscale;
-            s_v[idx] = in ? make_float2(fvis[i].x * vscale, fvis[i].y * vscale) : make_float2(0.f, 0.f);
+            w_a[idx] = in ? (g.u_off - fk[i] * fu[i]) * kInvTwoPi : 0.0;
+            w_b[idx] = in ? (g.v_off - fk[i] * fv[i]) * kInvTwoPi : 0.0;
+            w_c[idx] = cc;
+            w_cf[idx] = (float)cc * cscale;
+            w_v[idx] = in ? make_float2(fvis[i].x * vscale, fvis[i].y * vscale) : make_float2(0.f, 0.f);
         }
-        __syncthreads();
-        if (j0 + ROUND < K) fetch(j0 + ROUND);
-        const int nsteps = (min(ROUND, K - j0) + 31) / 32;
-        for (int st = wave; st < nsteps; st += 4) {
+        __builtin_amdgcn_wave_barrier();
+        if (j0 + WROUND < jw1) fetch(j0 + WROUND);
+        const int nsteps = (min(WROUND, jw1 - j0) + 31) / 32;
+        for (int st = 0; st < nsteps; st++) {
             const int jg = st * 32 + grp * 8;  // this lane group's 8 visibilities
             // x-phasors of the column tiles, split; -Ai carries the products'
             // minus signs (one negation per column tile, not three per row tile)
@@ -497,7 +510,7 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 float2 A[8];
-                phasors(jg, s_a, t, A);
+                phasors(jg, w_a, t, A);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     P9_SPLIT2(P, sk, A[2 * q].x, A[2 * q + 1].x, Arh[t][q], Arl[t][q]);
@@ -509,15 +522,15 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
             // c of the 8 visibilities (FP32; scaled for FP16)
             float cf[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) cf[e] = s_cf[jg + e];
+            for (int e = 0; e < 8; e++) cf[e] = w_cf[jg + e];
 #pragma unroll
             for (int r = 0; r < NT; r++) {
                 float2 B[8];
-                phasors(jg, s_b, r, B);
+                phasors(jg, w_b, r, B);
                 u32x4 Lrh, Lrl, Lih, Lil, cLr, cLi;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const float2 L0 = cmulf(s_v[jg + 2 * q], B[2 * q]), L1 = cmulf(s_v[jg + 2 * q + 1], B[2 * q + 1]);
+                    const float2 L0 = cmulf(w_v[jg + 2 * q], B[2 * q]), L1 = cmulf(w_v[jg + 2 * q + 1], B[2 * q + 1]);
                     cLr[q] = P::pack(cf[2 * q] * L0.x, cf[2 * q + 1] * L1.x);
                     cLi[q] = P::pack(cf[2 * q] * L0.y, cf[2 * q + 1] * L1.y);
                     P9_SPLIT2(P, sk, L0.x, L1.x, Lrh[q], Lrl[q]);
@@ -547,8 +560,9 @@ __device__ __forceinline__ void grid_mfma16_body(const KernelArgs& a, int sidx, 
                 }
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();  // this round's reads before the next round's stores
     }
+    __syncthreads();  // every wave is done with the staging arrays the partials overlay
 
     // subgrid = (M0 + 2 pi i r M1) / vscale, ifftshift-ed (Gridder.cs:76-79;
     // identity taper): each wave writes its part to its own tile, then the
