@@ -362,9 +362,10 @@ __device__ __forceinline__ void grid_mfma_body(const KernelArgs& a, int sidx, in
 // ---------------------------------------------------------------------------
 // Gridder for S = 16, 32, 48 on 16x16x32 MFMA tiles: the subgrid is NT x NT
 // tiles of 16 x 16 (S = 48 is not padded to 64).  One 256-thread workgroup
-// per work item; a step is 32 visibilities, the 4 waves take alternate steps
-// and each accumulates all NT x NT tiles.  Lane l = 16 g + c: c is the row (A operand) and column (B
-// operand) within a tile, g picks the lane's 8 visibilities of the step.
+// per work item; a step is 32 visibilities, each of the 4 waves takes a
+// contiguous quarter of the item's steps and accumulates all NT x NT tiles.
+// Lane l = 16 g + c: c is the row (A operand) and column (B operand) within a
+// tile, g picks the lane's 8 visibilities of the step.
 // Per step the x-phasors of the NT column tiles are built once, then each row
 // tile's V * y-phasor operands are built and multiplied into its NT tiles.
 // fresh: the plan's tables (rtab, twiddles, adder phasors) are not yet in
