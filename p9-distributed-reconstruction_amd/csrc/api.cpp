@@ -1592,11 +1592,26 @@ struct p9idg_batch {
     hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev[5] = {};
     bool timed = false;
+    // with several groups, each group's launch runs on a stream of its own
+    // between a fork and a join on `stream`, so one group's launch fills the
+    // CUs that another's tail leaves idle
+    hipEvent_t fork = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_done;
 };
 
 namespace {
 
 void batch_free_groups(p9idg_batch* b) {
+    for (size_t i = 0; i < b->side.size(); i++) {
+        if (b->side[i]) {
+            (void)hipStreamSynchronize(b->side[i]);
+            (void)hipStreamDestroy(b->side[i]);
+        }
+        if (b->side_done[i]) (void)hipEventDestroy(b->side_done[i]);
+    }
+    b->side.clear();
+    b->side_done.clear();
     for (auto& g : b->groups) {
         dev_free(g.d_args_grid);
         dev_free(g.d_args_degrid);
@@ -1638,6 +1653,31 @@ int batch_order_out(p9idg_batch* b) {
     return P9IDG_OK;
 }
 
+// One persistent launch per group, on the batch stream or fanned out over the
+// groups' streams.
+int batch_launch_groups(p9idg_batch* b, bool grid) {
+    const bool fan = b->side.size() == b->groups.size() && b->groups.size() > 1;
+    if (fan) P9_HIP(hipEventRecord(b->fork, b->stream));
+    for (size_t i = 0; i < b->groups.size(); i++) {
+        auto& g = b->groups[i];
+        hipStream_t st = fan ? b->side[i] : b->stream;
+        if (fan) P9_HIP(hipStreamWaitEvent(st, b->fork, 0));
+        hipError_t e;
+        if (grid) {
+            BatchArgs ba{g.d_args_grid, g.d_items_grid, g.n_grid, g.d_counter};
+            e = launch_grid_batch(ba, g.S, g.kernel, g.workgroups, st);
+        } else {
+            BatchArgs ba{g.d_args_degrid, g.d_items_degrid, g.n_degrid, g.d_counter};
+            e = launch_degrid_batch(ba, g.S, g.kernel, g.workgroups_degrid, st);
+        }
+        if (e != hipSuccess) return hip_fail(e, grid ? "batched gridder launch" : "batched degridder launch");
+        if (fan) P9_HIP(hipEventRecord(b->side_done[i], st));
+    }
+    if (fan)
+        for (size_t i = 0; i < b->groups.size(); i++) P9_HIP(hipStreamWaitEvent(b->stream, b->side_done[i], 0));
+    return P9IDG_OK;
+}
+
 // Zero fills and one persistent launch per (subgrid size, kernel) group.
 int batch_run(p9idg_batch* b, int what, bool capturing) {
     if (int st = batch_check(b)) return st;
@@ -1650,21 +1690,13 @@ int batch_run(p9idg_batch* b, int what, bool capturing) {
             P9_HIP(launch_fill_zero(target, n, b->stream));
         }
         if (!capturing) P9_HIP(hipEventRecord(b->ev[1], b->stream));
-        for (auto& g : b->groups) {
-            BatchArgs ba{g.d_args_grid, g.d_items_grid, g.n_grid, g.d_counter};
-            hipError_t e = launch_grid_batch(ba, g.S, g.kernel, g.workgroups, b->stream);
-            if (e != hipSuccess) return hip_fail(e, "batched gridder launch");
-        }
+        if (int st = batch_launch_groups(b, true)) return st;
     }
     if (!capturing) P9_HIP(hipEventRecord(b->ev[2], b->stream));
     if (what & 2) {
         for (p9idg_plan* p : b->plans) P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), b->stream));
         if (!capturing) P9_HIP(hipEventRecord(b->ev[3], b->stream));
-        for (auto& g : b->groups) {
-            BatchArgs ba{g.d_args_degrid, g.d_items_degrid, g.n_degrid, g.d_counter};
-            hipError_t e = launch_degrid_batch(ba, g.S, g.kernel, g.workgroups_degrid, b->stream);
-            if (e != hipSuccess) return hip_fail(e, "batched degridder launch");
-        }
+        if (int st = batch_launch_groups(b, false)) return st;
     }
     if (!capturing) {
         P9_HIP(hipEventRecord(b->ev[4], b->stream));
@@ -1697,7 +1729,8 @@ int p9idg_batch_create(p9idg_plan* const* plans, int32_t n, p9idg_batch** out) {
     for (auto& ev : b->ev) (void)hipEventCreate(&ev);
     b->plan_ev.assign(n, nullptr);
     b->obs_gen.assign(n, 0);
-    bool ok = hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    bool ok = hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
     for (auto& ev : b->plan_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         p9idg_batch_destroy(b);
@@ -1717,6 +1750,7 @@ int p9idg_batch_destroy(p9idg_batch* b) {
     for (auto& ev : b->plan_ev)
         if (ev) (void)hipEventDestroy(ev);
     if (b->done) (void)hipEventDestroy(b->done);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return P9IDG_OK;
@@ -1788,6 +1822,19 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
         if (!id.empty()) P9_HIP(hipMemcpy(g.d_items_degrid, id.data(), id.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
         g.n_grid = (int32_t)ig.size();
         g.n_degrid = (int32_t)id.size();
+    }
+    if (b->groups.size() > 1) {
+        b->side.assign(b->groups.size(), nullptr);
+        b->side_done.assign(b->groups.size(), nullptr);
+        for (size_t i = 0; i < b->groups.size(); i++) {
+            if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) != hipSuccess) b->side[i] = nullptr;
+            if (hipEventCreateWithFlags(&b->side_done[i], hipEventDisableTiming) != hipSuccess) b->side_done[i] = nullptr;
+            if (!b->side[i] || !b->side_done[i]) {
+                (void)hipGetLastError();
+                batch_free_groups(b);  // also drops the streams created so far
+                return fail(P9IDG_ERR_HIP, "batch stream creation failed");
+            }
+        }
     }
     for (size_t i = 0; i < b->plans.size(); i++) b->obs_gen[i] = b->plans[i]->obs_gen;
     b->prepared = true;
