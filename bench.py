@@ -15,6 +15,8 @@ Prints one JSON line.  Beside the headline it carries:
                   form), each autotuned, timed in this run
   hbm             achieved GB/s of the HBM-bound stages (target zeroing,
                   ToImage, ToUVW), HIP events over repeated runs
+  roofline        the slower direction's kernel against the MFMA (or FP32)
+                  peak; roofline.other: the other direction's kernel
   error_vs_oracle the timed plan (its autotuned launch shape) against the FP64
                   oracle on the whole workload: dirty image, PSF, degridded
                   visibilities (max and RMS relative error)
@@ -708,6 +710,11 @@ def main():
         # 32 visibilities, where the complex multiply-add needs the work of 4
         roofline["mfma_issued_per_algorithmic"] = 4.0
         roofline["issued_frac"] = 4.0 * achieved / peak
+    # the other direction's kernel against the same peak (same algorithmic work)
+    oth_name, oth_ms = ("degridder", avg_d) if dom_name == "gridder" else ("gridder", avg_g)
+    oth = flops_per_launch / (oth_ms / 1e3) / 1e12
+    roofline["other"] = {"kernel": oth_name, "achieved": oth, "frac": oth / peak, "kernel_ms": oth_ms,
+                         "traffic": profiled_traffic(args.kernel, oth_name)[0]}
     cpu, err = None, None
     if not args.no_oracle:
         cpu, err = oracle_checks(ob, part, gpu, model_grid)
