@@ -862,7 +862,10 @@ int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
     if ((st = dev_alloc(&p->d_grid, (size_t)G * G))) return bail(st);
     if ((st = dev_alloc(&p->d_model_grid, (size_t)G * G))) return bail(st);
     if ((st = dev_alloc(&p->d_image, (size_t)G * G))) return bail(st);
-    if ((st = dev_alloc(&p->d_counter, (size_t)1))) return bail(st);
+    if ((st = dev_alloc(&p->d_counter, (size_t)2))) return bail(st);
+    // the queue kernels leave their counters at zero; they start there
+    e = launch_reset_counter(p->d_counter, p->stream);
+    if (e != hipSuccess) return bail(hip_fail(e, "counter reset"));
     e = hipMemcpy(p->d_twiddle, tw.data(), S * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(hip_fail(e, "twiddle upload"));
     e = hipMemcpy(p->d_phasor, ph.data(), ph.size() * sizeof(float2), hipMemcpyHostToDevice);
@@ -1814,8 +1817,9 @@ int p9idg_batch_prepare(p9idg_batch* b, int32_t grid_mode, int32_t degrid_mode) 
         int st;
         if ((st = dev_alloc(&g.d_args_grid, ag.size())) || (st = dev_alloc(&g.d_args_degrid, ad.size())) ||
             (st = dev_alloc(&g.d_items_grid, std::max<size_t>(ig.size(), 1))) ||
-            (st = dev_alloc(&g.d_items_degrid, std::max<size_t>(id.size(), 1))) || (st = dev_alloc(&g.d_counter, (size_t)1)))
+            (st = dev_alloc(&g.d_items_degrid, std::max<size_t>(id.size(), 1))) || (st = dev_alloc(&g.d_counter, (size_t)2)))
             return st;
+        P9_HIP(launch_reset_counter(g.d_counter, b->stream));  // the queue kernels leave it at zero
         P9_HIP(hipMemcpy(g.d_args_grid, ag.data(), ag.size() * sizeof(KernelArgs), hipMemcpyHostToDevice));
         P9_HIP(hipMemcpy(g.d_args_degrid, ad.data(), ad.size() * sizeof(KernelArgs), hipMemcpyHostToDevice));
         if (!ig.empty()) P9_HIP(hipMemcpy(g.d_items_grid, ig.data(), ig.size() * sizeof(WorkItem), hipMemcpyHostToDevice));

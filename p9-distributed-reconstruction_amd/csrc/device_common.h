@@ -213,9 +213,20 @@ __device__ __forceinline__ float2* fft_2d(float2* a, float2* b, const float2* tw
     return src;
 }
 
-// Persistent work queue: each workgroup takes the next WorkItem from the
-// counter until every item is taken; body(sidx, k0, k1) processes one.  All
-// threads of the workgroup leave together once the counter passes nitems.
+// Persistent work queues.  counter[0] hands out item indices, counter[1]
+// counts the workgroups that found the queue empty; the last of them sets both
+// back to zero, so consecutive launches on the same counters need no reset
+// (every claim of a workgroup returns before it counts itself out).
+__device__ __forceinline__ void queue_exit(int32_t* counter) {
+    if (threadIdx.x == 0 && atomicAdd(counter + 1, 1) == (int)gridDim.x - 1) {
+        __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(counter + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Each workgroup takes the next WorkItem from the counter until every item is
+// taken; body(sidx, k0, k1) processes one.  All threads of the workgroup
+// leave together once the counter passes nitems.
 // AHEAD: the next item is claimed as soon as the current one is known, so the
 // counter's round trip overlaps the current item's work; the workgroup then
 // holds one claimed item while it works, which can lengthen the tail (on the
@@ -235,17 +246,18 @@ __device__ __forceinline__ void run_queue(const KernelArgs& a, Body body) {
             __syncthreads();  // this item's LDS reads are done; s_item is the next
             it = s_item;
         }
-        return;
+    } else {
+        for (;;) {
+            __syncthreads();  // previous item's LDS reads are done
+            if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+            __syncthreads();
+            const int it = s_item;
+            if (it >= a.nitems) break;
+            const WorkItem w = a.items[it];
+            body(w.sidx, w.k0, w.k1, w.slot);
+        }
     }
-    for (;;) {
-        __syncthreads();  // previous item's LDS reads are done
-        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
-        __syncthreads();
-        const int it = s_item;
-        if (it >= a.nitems) return;
-        const WorkItem w = a.items[it];
-        body(w.sidx, w.k0, w.k1, w.slot);
-    }
+    queue_exit(a.counter);
 }
 
 // Batched queue: like run_queue, with each item's observation in args[inst].
@@ -265,18 +277,19 @@ __device__ __forceinline__ void run_batch_queue(const BatchArgs& b, Body body) {
             __syncthreads();  // this item's LDS reads are done; s_item is the next
             it = s_item;
         }
-        return;
+    } else {
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) s_item = atomicAdd(b.counter, 1);
+            __syncthreads();
+            const int it = s_item;
+            if (it >= b.nitems) break;
+            const WorkItem w = b.items[it];
+            const KernelArgs a = b.args[w.inst];
+            body(a, w.sidx, w.k0, w.k1, w.slot);
+        }
     }
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_item = atomicAdd(b.counter, 1);
-        __syncthreads();
-        const int it = s_item;
-        if (it >= b.nitems) return;
-        const WorkItem w = b.items[it];
-        const KernelArgs a = b.args[w.inst];
-        body(a, w.sidx, w.k0, w.k1, w.slot);
-    }
+    queue_exit(b.counter);
 }
 
 // Linear index of grid element (y, x) after FFT.Shift (FFT.cs:248-264): the

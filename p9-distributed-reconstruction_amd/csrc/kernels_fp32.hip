@@ -361,7 +361,11 @@ __global__ __launch_bounds__(kThreads) void degrid_fp32_batch_kernel(BatchArgs b
     run_batch_queue(b, [&](const KernelArgs& a, int s, int k0, int k1, int) { degrid_fp32_any<S>(a, s, k0, k1, SEP); });
 }
 
-__global__ void reset_counter_kernel(int32_t* counter) { *counter = 0; }
+// both queue counters (claims and exits) to zero
+__global__ void reset_counter_kernel(int32_t* counter) {
+    counter[0] = 0;
+    counter[1] = 0;
+}
 
 }  // namespace dev
 
@@ -374,7 +378,6 @@ __global__ void reset_counter_kernel(int32_t* counter) { *counter = 0; }
                                         : persistent_workgroups((const void*)QKERNEL<SV, SEPV>, dev::kThreads); \
             wg = wg < a.nitems ? wg : a.nitems;                                               \
             if (wg <= 0) return hipSuccess;                                                   \
-            hipLaunchKernelGGL(dev::reset_counter_kernel, dim3(1), dim3(1), 0, st, a.counter); \
             hipLaunchKernelGGL((QKERNEL<SV, SEPV>), dim3(wg), dim3(dev::kThreads), 0, st, a); \
         } else {                                                                              \
             hipLaunchKernelGGL((KERNEL<SV, SEPV>), dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); \
@@ -423,7 +426,6 @@ hipError_t fp32_batch_one(bool grid, const BatchArgs& b, int workgroups, hipStre
     int wg = workgroups > 0 ? workgroups : persistent_workgroups(k, dev::kThreads);
     wg = wg < b.nitems ? wg : b.nitems;
     if (wg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(dev::reset_counter_kernel, dim3(1), dim3(1), 0, st, b.counter);
     if (grid)
         hipLaunchKernelGGL((dev::grid_fp32_batch_kernel<S, SEP>), dim3(wg), dim3(dev::kThreads), 0, st, b);
     else

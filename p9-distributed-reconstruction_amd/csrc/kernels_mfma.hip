@@ -1102,8 +1102,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDegri
                          : persistent_workgroups((const void*)QKERNEL<SV, PREC>, dev::kThreads);   \
             wg = wg < a.nitems ? wg : a.nitems;                                                    \
             if (wg <= 0) return hipSuccess;                                                        \
-            hipError_t e_ = launch_reset_counter(a.counter, st);                                   \
-            if (e_ != hipSuccess) return e_;                                                       \
             hipLaunchKernelGGL((QKERNEL<SV, PREC>), dim3(wg), dim3(dev::kThreads), 0, st, a);      \
         } else {                                                                                   \
             hipLaunchKernelGGL((KERNEL<SV, PREC>), dim3(a.nr_subgrids), dim3(dev::kThreads), 0, st, a); \
@@ -1140,8 +1138,6 @@ hipError_t launch_batch_kernel(K kernel, const BatchArgs& b, int workgroups, hip
     int wg = workgroups > 0 ? workgroups : persistent_workgroups((const void*)kernel, dev::kThreads);
     wg = wg < b.nitems ? wg : b.nitems;
     if (wg <= 0) return hipSuccess;
-    hipError_t e = launch_reset_counter(b.counter, st);
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kernel, dim3(wg), dim3(dev::kThreads), 0, st, b);
     return hipGetLastError();
 }

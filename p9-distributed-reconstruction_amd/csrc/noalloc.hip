@@ -50,7 +50,7 @@ Layout layout(int S, int C, int nsub) {
     Layout l{};
     size_t o = 0;
     l.counter = o;
-    o = align256(o + sizeof(int32_t));
+    o = align256(o + 2 * sizeof(int32_t));
     l.twiddle = o;
     o = align256(o + sizeof(float2) * S);
     l.phasor = o;
@@ -187,7 +187,7 @@ extern "C" int p9idg_grid_noalloc(const p9idg_constants* c, int32_t kernel, int3
     a.split_k[2] = 0x0000BC00u;
     a.items = reinterpret_cast<const WorkItem*>(base + lay.items);
     a.nitems = nsub;
-    a.counter = reinterpret_cast<int32_t*>(base + lay.counter);  // zeroed by the launcher's reset kernel
+    a.counter = reinterpret_cast<int32_t*>(base + lay.counter);  // zeroed below: the array keeps the last call's
     // two persistent workgroups per CU (the residency of the S <= 32
     // kernels; larger subgrids leave the extra ones to find the queue empty),
     // from the device attribute rather than the launcher's occupancy cache,
@@ -197,6 +197,7 @@ extern "C" int p9idg_grid_noalloc(const p9idg_constants* c, int32_t kernel, int3
     LaunchCfg cfg;
     cfg.queue = true;
     cfg.workgroups = 2 * (cus > 0 ? cus : 1);
+    P9_HIP(launch_reset_counter(a.counter, st));
     hipError_t e;
     if (kernel == P9IDG_KERNEL_FP32 || kernel == P9IDG_KERNEL_FP32_SEP)
         e = launch_grid_fp32(a, S, kernel == P9IDG_KERNEL_FP32_SEP, cfg, st);

@@ -119,7 +119,7 @@ struct KernelArgs {
     uint32_t split_k[3];
     const WorkItem* items; // work queue (longest first)
     int32_t nitems;
-    int32_t* counter;      // next item; zeroed by the launcher before each launch
+    int32_t* counter;      // [2]: next item, finished workgroups; zero between launches
 };
 
 // Several observations in one persistent launch: item.inst indexes args.
@@ -127,7 +127,7 @@ struct BatchArgs {
     const KernelArgs* args;
     const WorkItem* items;
     int32_t nitems;
-    int32_t* counter;
+    int32_t* counter;  // [2], as in KernelArgs
 };
 
 // Launchers (kernels_*.hip); S is the subgrid size.
@@ -138,6 +138,8 @@ hipError_t launch_grid_mfma(const KernelArgs& a, int S, int precision, const Lau
                             hipStream_t st);
 hipError_t launch_degrid_mfma(const KernelArgs& a, int S, int precision, const LaunchCfg& cfg,
                               hipStream_t st);
+// both queue counters to zero (once per allocation: the queue kernels leave
+// them at zero)
 hipError_t launch_reset_counter(int32_t* counter, hipStream_t st);
 hipError_t launch_rtab(float* rtab, int S, float image_size, hipStream_t st);
 hipError_t launch_fp32_batch(bool grid, bool sep, const BatchArgs& b, int S, int workgroups, hipStream_t st);
