@@ -135,6 +135,10 @@ struct p9idg_plan {
     bool have_fft_one = false;
 
     hipEvent_t ev[8] = {};
+    // the event marking timing slot i for p9idg_last_timings: ev[i], or the
+    // span event recorded at that position while spans are collected
+    hipEvent_t mark[8] = {};
+    hipEvent_t last_span_event = nullptr;  // the latest event a span recorded
     bool timed_grid = false, timed_degrid = false, timed_fft = false;
 
     // Event pairs of every gridder / degridder launch since the last reset
@@ -320,10 +324,18 @@ hipEvent_t pooled_event(p9idg_plan* p, size_t i) {
     return p->event_pool[i];
 }
 
-// Records a plan event unless the stream is being captured into a graph.
+// Timing slot i of p9idg_last_timings at the current stream position (not
+// while capturing).  It must directly follow a span call when spans are
+// collected: each event on the stream delays the next launch, so a position
+// records one event, which the span and the slot share.
 int record(p9idg_plan* p, int i) {
     if (p->capturing) return P9IDG_OK;
+    if (p->collect && p->last_span_event) {
+        p->mark[i] = p->last_span_event;
+        return P9IDG_OK;
+    }
     P9_HIP(hipEventRecord(p->ev[i], p->stream));
+    p->mark[i] = p->ev[i];
     return P9IDG_OK;
 }
 
@@ -335,11 +347,26 @@ int span_begin(p9idg_plan* p, int kind) {
     if (!a || !b) return fail(P9IDG_ERR_HIP, "event creation failed");
     P9_HIP(hipEventRecord(a, p->stream));
     p->spans.push_back({a, b, kind});
+    p->last_span_event = a;
     return P9IDG_OK;
 }
 int span_end(p9idg_plan* p) {
     if (!p->collect || p->capturing || p->spans.empty()) return P9IDG_OK;
     P9_HIP(hipEventRecord(p->spans.back().stop, p->stream));
+    p->last_span_event = p->spans.back().stop;
+    return P9IDG_OK;
+}
+// Ends the open span and starts the next one at the same position, with one
+// event for both.
+int span_next(p9idg_plan* p, int kind) {
+    if (!p->collect || p->capturing) return P9IDG_OK;
+    if (p->spans.empty()) return span_begin(p, kind);
+    hipEvent_t b = pooled_event(p, p->spans.size() * 2 + 1);
+    if (!b) return fail(P9IDG_ERR_HIP, "event creation failed");
+    hipEvent_t e = p->spans.back().stop;
+    P9_HIP(hipEventRecord(e, p->stream));
+    p->spans.push_back({e, b, kind});
+    p->last_span_event = e;
     return P9IDG_OK;
 }
 
@@ -569,12 +596,10 @@ int do_exec_grid(p9idg_plan* p, int mode) {
         if (int st = prepare_gather(p, mode)) return st;
         a.scratch = p->d_scratch;
     }
-    if (int st_ = record(p, 0)) return st_;
     if (int st = span_begin(p, P9IDG_STAGE_ZERO)) return st;
     if (int st = zero_target(p, mode, target, n)) return st;
-    if (int st = span_end(p)) return st;
+    if (int st = span_next(p, P9IDG_STAGE_GRIDDER)) return st;
     if (int st_ = record(p, 1)) return st_;
-    if (int st = span_begin(p, P9IDG_STAGE_GRIDDER)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32 || p->kernel == P9IDG_KERNEL_FP32_SEP)
         e = launch_grid_fp32(a, p->c.subgrid_size, p->kernel == P9IDG_KERNEL_FP32_SEP, launch_cfg(p, true),
@@ -607,11 +632,12 @@ int do_exec_degrid(p9idg_plan* p, int mode) {
     if (p->capturing || !p->model_vis_clean) {
         if (int st = span_begin(p, P9IDG_STAGE_ZERO)) return st;
         P9_HIP(launch_fill_zero(p->d_model_vis, vis_elems(p), p->stream));
-        if (int st = span_end(p)) return st;
+        if (int st = span_next(p, P9IDG_STAGE_DEGRIDDER)) return st;
+    } else {
+        if (int st = span_begin(p, P9IDG_STAGE_DEGRIDDER)) return st;
     }
     p->model_vis_clean = !p->capturing;
     if (int st_ = record(p, 3)) return st_;
-    if (int st = span_begin(p, P9IDG_STAGE_DEGRIDDER)) return st;
     hipError_t e;
     if (p->kernel == P9IDG_KERNEL_FP32 || p->kernel == P9IDG_KERNEL_FP32_SEP)
         e = launch_degrid_fp32(a, p->c.subgrid_size, p->kernel == P9IDG_KERNEL_FP32_SEP, launch_cfg(p, false),
@@ -655,8 +681,8 @@ int do_grid_to_image(p9idg_plan* p, int mode) {
     DeviceGuard dg(p->device);
     const int G = p->c.grid_size;
     const double inv_n = 1.0 / (double)p->c.visibilities_count;
-    if (int st_ = record(p, 5)) return st_;
     if (int st = span_begin(p, P9IDG_STAGE_TO_IMAGE)) return st;
+    if (int st_ = record(p, 5)) return st_;
     if (mode == P9IDG_GRID_WSTACK) {
         if (!p->d_layers) return fail(P9IDG_ERR_STATE, "no layers gridded");
         if (int st = layers_image(p, inv_n, 1)) return st;
@@ -843,6 +869,7 @@ int p9idg_create(const p9idg_constants* c, int32_t device, p9idg_plan** plan) {
         e = hipEventCreate(&ev);
         if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
     }
+    for (int i = 0; i < 8; i++) p->mark[i] = p->ev[i];
     const int S = c->subgrid_size, G = c->grid_size;
     // tables: twiddles exp(+2 pi i m / S) and the adder phasor
     // exp(i pi (x + y - S) / S) (Adder.cs:37-40, AdderWStack.cs:17-25), FP64 -> float
@@ -1345,7 +1372,7 @@ static int time_direction(p9idg_plan* p, int dir, int reps, float* ms_out) {
         if (st) return st;
         P9_HIP(hipStreamSynchronize(p->stream));
         float ms = 0.f;
-        P9_HIP(hipEventElapsedTime(&ms, p->ev[dir == 0 ? 1 : 3], p->ev[dir == 0 ? 2 : 4]));
+        P9_HIP(hipEventElapsedTime(&ms, p->mark[dir == 0 ? 1 : 3], p->mark[dir == 0 ? 2 : 4]));
         t.push_back(ms);
     }
     std::sort(t.begin(), t.end());
@@ -1506,15 +1533,15 @@ int p9idg_last_timings(p9idg_plan* p, float* grid_ms, float* degrid_ms, float* f
     float ms = 0.f;
     if (grid_ms) {
         *grid_ms = -1.f;
-        if (p->timed_grid && hipEventElapsedTime(&ms, p->ev[1], p->ev[2]) == hipSuccess) *grid_ms = ms;
+        if (p->timed_grid && hipEventElapsedTime(&ms, p->mark[1], p->mark[2]) == hipSuccess) *grid_ms = ms;
     }
     if (degrid_ms) {
         *degrid_ms = -1.f;
-        if (p->timed_degrid && hipEventElapsedTime(&ms, p->ev[3], p->ev[4]) == hipSuccess) *degrid_ms = ms;
+        if (p->timed_degrid && hipEventElapsedTime(&ms, p->mark[3], p->mark[4]) == hipSuccess) *degrid_ms = ms;
     }
     if (fft_ms) {
         *fft_ms = -1.f;
-        if (p->timed_fft && hipEventElapsedTime(&ms, p->ev[5], p->ev[6]) == hipSuccess) *fft_ms = ms;
+        if (p->timed_fft && hipEventElapsedTime(&ms, p->mark[5], p->mark[6]) == hipSuccess) *fft_ms = ms;
     }
     return P9IDG_OK;
 }
@@ -1525,6 +1552,11 @@ int p9idg_timing_reset(p9idg_plan* p, int32_t collect) {
     P9_HIP(hipStreamSynchronize(p->stream));
     p->spans.clear();
     p->collect = collect != 0;
+    // the pooled span events are recorded afresh from here on: drop the
+    // timing slots that pointed at them
+    p->last_span_event = nullptr;
+    for (int i = 0; i < 8; i++) p->mark[i] = p->ev[i];
+    p->timed_grid = p->timed_degrid = p->timed_fft = false;
     return P9IDG_OK;
 }
 
