@@ -260,11 +260,14 @@ int p9idg_upload_image(p9idg_plan* plan, const float* img);
 int p9idg_download_image(p9idg_plan* plan, float* out);
 
 /* Milliseconds of the last exec_grid / exec_degrid kernel (HIP events on the
- * plan's stream) and of the last FFT stage. */
+ * plan's stream) and of the last FFT stage since the last p9idg_timing_reset;
+ * -1 for a stage not run since. */
 int p9idg_last_timings(p9idg_plan* plan, float* grid_ms, float* degrid_ms, float* fft_ms);
 /* Collect HIP-event spans around every gridder / degridder kernel launch
- * (collect != 0) from now on, dropping earlier ones; totals are summed over
- * the collected spans. */
+ * (collect != 0) from now on, dropping earlier ones and the last timings;
+ * totals are summed over the collected spans.  Adjacent spans share their
+ * boundary event (one event per stream position), so a span includes the
+ * launch gap before its kernel. */
 int p9idg_timing_reset(p9idg_plan* plan, int32_t collect);
 int p9idg_timing_totals(p9idg_plan* plan, double* grid_ms, int64_t* grid_launches,
                         double* degrid_ms, int64_t* degrid_launches);
