@@ -476,6 +476,44 @@ def test_fft_steps(O):
     assert rel_err(p9idg.FFT.Backward(grid, N), O.fft_backward(grid, N)) <= TOL
 
 
+def test_timing_spans_and_last_timings(small):
+    """HIP-event timing: spans of every gridder / degridder launch while
+    collecting (adjacent spans share one event per stream position), the
+    last-timing slots in both modes, and a reset that clears them."""
+    ob, part, _ = small
+    G = ob.c.GridSize
+    with p9idg.Plan(ob.c, kernel="mfma_bf16") as plan:
+        plan.set_observation(ob.uvw, ob.freq, part)
+        plan.upload_vis(ob.vis)
+        plan.upload_model_grid(np.ones((G, G), dtype=np.complex64))
+        plan.timing_reset(True)
+        for _ in range(3):
+            plan.exec_grid(p9idg.GRID_WSTACK)
+            plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        plan.exec_grid_to_image(p9idg.GRID_WSTACK)
+        plan.sync()
+        g_ms, g_n, d_ms, d_n = plan.timing_totals()
+        assert (g_n, d_n) == (3, 3) and g_ms > 0 and d_ms > 0
+        z_ms, z_n = plan.timing_stage(p9idg.STAGE_ZERO)
+        assert z_n >= 3 and z_ms >= 0
+        ti_ms, ti_n = plan.timing_stage(p9idg.STAGE_TO_IMAGE)
+        assert ti_n == 1 and ti_ms > 0
+        t = plan.timings()
+        assert t["grid_ms"] > 0 and t["degrid_ms"] > 0 and t["fft_ms"] > 0
+        # the last spans and the last-timing slots are the same events
+        assert t["grid_ms"] <= g_ms and t["degrid_ms"] <= d_ms
+        assert abs(t["fft_ms"] - ti_ms) <= 1e-3 * max(ti_ms, 1e-3)
+        plan.timing_reset(False)
+        t = plan.timings()
+        assert (t["grid_ms"], t["degrid_ms"], t["fft_ms"]) == (-1.0, -1.0, -1.0)
+        plan.exec_grid(p9idg.GRID_WSTACK)
+        plan.exec_degrid(p9idg.DEGRID_WSTACK)
+        plan.sync()
+        t = plan.timings()
+        assert t["grid_ms"] > 0 and t["degrid_ms"] > 0 and t["fft_ms"] == -1.0
+        assert plan.timing_totals()[1] == 0  # not collecting
+
+
 def test_c64_and_c128_inputs_agree(small):
     ob, part, oc = small
     with p9idg.Plan(ob.c) as plan:
